@@ -1,0 +1,146 @@
+/*
+ * impes_hip.h — C ABI of the MI355X (gfx950) IMPES pressure/transport library.
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative code on failure;
+ *     impes_last_error() returns the message of the calling thread's last failure;
+ *   - pointers named like arrays are DEVICE pointers owned by the caller (the
+ *     library only borrows them for the call), except where marked "host";
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); all work
+ *     is enqueued on it; functions that return host values synchronise it;
+ *   - indices are int32, values double (FP64) or float (FP32);
+ *   - opaque handles own their device memory until *_destroy.
+ *
+ * Each entry point cites the reference interface (researchim-ai/oil-simulator-py)
+ * whose role it takes over.
+ */
+#ifndef IMPES_HIP_H
+#define IMPES_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMPES_ABI_VERSION 1
+
+typedef struct impes_amg impes_amg;
+typedef struct impes_solver impes_solver;
+typedef struct impes_model impes_model;
+
+const char* impes_last_error(void);
+int impes_abi_version(void);
+
+/* ------------------------------------------------------------------ operators */
+
+/* y = A x (CSR). Replaces ClassicalAMG._spmv_csr(A, x, transpose=False),
+ * src/solver/classical_amg.py:976-1009 (gather + scatter_add per call). */
+int impes_spmv_csr_f64(int n_rows, int n_cols, int nnz, const int* rowptr, const int* col, const double* val,
+                       const double* x, double* y, void* stream);
+int impes_spmv_csr_f32(int n_rows, int n_cols, int nnz, const int* rowptr, const int* col, const float* val,
+                       const float* x, float* y, void* stream);
+
+/* y = A x for the 7-point pressure stencil in face form (diag d, face coefficients
+ * ox/oy/oz stored at the lower cell of each face, z fastest). Replaces the sparse
+ * product with the COO matrix of Simulator._build_pressure_matrix_vectorized,
+ * src/simulator/simulation.py:1546-1580. */
+int impes_spmv_st7_f64(int nx, int ny, int nz, const double* d, const double* ox, const double* oy,
+                       const double* oz, const double* x, double* y, void* stream);
+
+/* x_out = x_in + omega * inv_relax .* (b - A x_in): one fused L1-Jacobi sweep.
+ * Replaces ClassicalAMG._smooth, src/solver/classical_amg.py:1016-1035. */
+int impes_l1jacobi_csr_f64(int n, int nnz, const int* rowptr, const int* col, const double* val,
+                           const double* inv_relax, double omega, const double* x_in, const double* b,
+                           double* x_out, void* stream);
+int impes_l1jacobi_csr_f32(int n, int nnz, const int* rowptr, const int* col, const float* val,
+                           const float* inv_relax, float omega, const float* x_in, const float* b, float* x_out,
+                           void* stream);
+
+/* out[0] = x . y (device scalar, FP64 accumulation, deterministic two-stage
+ * reduction). Replaces the torch.dot / norm calls of the solvers,
+ * src/solver/classical_amg.py:1136-1142, src/simulator/simulation.py:1677-1689. */
+int impes_dot_f64(long long n, const double* x, const double* y, double* out, void* stream);
+
+/* ------------------------------------------------------------------ AMG hierarchy */
+
+typedef struct {
+  double theta;          /* strength threshold (reference default 0.25) */
+  int max_levels;        /* reference max_levels (hierarchy holds up to max_levels+1 levels) */
+  int coarsest_size;     /* stop coarsening at n <= coarsest_size */
+  int pre_smooth;        /* V-cycle pre-sweeps (3) */
+  int post_smooth;       /* V-cycle post-sweeps (2) */
+  double omega;          /* L1-Jacobi damping (0.7) */
+  double equil_threshold;/* equilibrate when the median |a_ii| < threshold (1e-6) */
+  int dense_max;         /* coarsest level solved densely up to this size (500) */
+  int coarse_sweeps;     /* Jacobi sweeps on a larger coarsest level (50) */
+  int anchor;            /* forced C point on every level, -1 = none (reference anchor_idx) */
+} impes_amg_params;
+
+void impes_amg_default_params(impes_amg_params* p);
+
+/* Replaces ClassicalAMG.__init__ (hierarchy build), src/solver/classical_amg.py:699-938. */
+int impes_amg_create(const impes_amg_params* params, impes_amg** out);
+int impes_amg_destroy(impes_amg* amg);
+int impes_amg_setup_csr(impes_amg* amg, int n, int nnz, const int* rowptr, const int* col, const double* val,
+                        void* stream);
+/* Same, from the 7-point stencil; level 0 then runs index-free. */
+int impes_amg_setup_st7(impes_amg* amg, int nx, int ny, int nz, const double* d, const double* ox,
+                        const double* oy, const double* oz, void* stream);
+/* New values, same pattern. Replaces ClassicalAMG.update_matrix,
+ * src/solver/classical_amg.py:1308-1392. */
+int impes_amg_update_csr(impes_amg* amg, const double* val, void* stream);
+int impes_amg_update_st7(impes_amg* amg, const double* d, const double* ox, const double* oy, const double* oz,
+                         void* stream);
+
+/* Introspection (host outputs): ClassicalAMG.levels, classical_amg.py:891-904. */
+int impes_amg_num_levels(const impes_amg* amg, int* out);
+int impes_amg_level_info(const impes_amg* amg, int level, int* n, int* nnz, int* p_nnz, int* is_coarsest,
+                         int* dense);
+int impes_amg_equilibrated(const impes_amg* amg, int* out);
+int impes_amg_setup_seconds(const impes_amg* amg, double* out);
+/* Copies level data into caller device buffers (sizes from level_info; any pointer may be NULL). */
+int impes_amg_export_level(const impes_amg* amg, int level, int* rowptr, int* col, double* val, int* p_rowptr,
+                           int* p_col, double* p_val, signed char* cf, double* inv_relax, void* stream);
+/* dhalf_inv = D^-1/2 of the fine matrix when equilibrated, ones otherwise
+ * (reference ClassicalAMG.Dhalf_inv, classical_amg.py:774-788). */
+int impes_amg_export_scaling(const impes_amg* amg, double* dhalf_inv, void* stream);
+
+/* One V(pre,post) cycle in the equilibrated space, x <- V(b); precision 64 or 32
+ * (32: b and x are FP64 buffers, converted at the boundary).
+ * Replaces ClassicalAMG._v_cycle, src/solver/classical_amg.py:1037-1106. */
+int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero, void* stream);
+/* Preconditioner of the row-scaled system D^-1 A: z = S V(S D v), S = D^-1/2 or I.
+ * Replaces the apply closure of amg_preconditioner, src/solver/pressure_amg.py:155-158. */
+int impes_amg_precondition(impes_amg* amg, int precision, const double* v, double* z, void* stream);
+
+/* ------------------------------------------------------------------ FGMRES */
+
+typedef struct {
+  int iterations;
+  int restarts;
+  int converged;
+  int precision;
+  double rel_residual; /* ||D^-1 (b - A x)|| / ||D^-1 b|| at exit */
+  double bnorm;        /* ||D^-1 b|| */
+  double r0_rel;       /* relative residual of the initial guess */
+} impes_solve_stats;
+
+int impes_solver_create(impes_solver** out);
+int impes_solver_destroy(impes_solver* solver);
+
+/* FP64 restarted FGMRES on D^-1 A x = D^-1 b, right-preconditioned by one AMG
+ * V-cycle (amg may be NULL: unpreconditioned). dinv = 1/diag(A). Takes over
+ * amg_solve, src/solver/pressure_amg.py:35-89, whose stationary V-cycle
+ * iteration it replaces by a Krylov method. stats is a host pointer. */
+int impes_fgmres_csr(impes_solver* solver, impes_amg* amg, int precision, int n, int nnz, const int* rowptr,
+                     const int* col, const double* val, const double* dinv, const double* b, double* x,
+                     int use_x0, double rtol, int restart, int maxit, impes_solve_stats* stats, void* stream);
+int impes_fgmres_st7(impes_solver* solver, impes_amg* amg, int precision, int nx, int ny, int nz,
+                     const double* d, const double* ox, const double* oy, const double* oz, const double* dinv,
+                     const double* b, double* x, int use_x0, double rtol, int restart, int maxit,
+                     impes_solve_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMPES_HIP_H */

@@ -1,0 +1,132 @@
+// Ruge-Stueben AMG hierarchy living in device memory.
+//
+// Mirrors ClassicalAMG of src/solver/classical_amg.py:689-1392: optional
+// symmetric equilibration of the fine matrix, RS coarsening with RS + F-F
+// interpolation, Galerkin coarse operators, L1-Jacobi V(3,2) cycles with
+// omega = 0.7, and a dense DOF-0-pinned coarsest solve. Two value copies of
+// every level operator are kept: FP64 and FP32 (the FP32 hierarchy is the
+// preconditioner of the FP64 outer Krylov solver).
+#pragma once
+
+#include <vector>
+
+#include "kernels.h"
+
+namespace impes {
+
+struct AmgParams {
+  double theta = 0.25;
+  int max_levels = 10;
+  int coarsest_size = 200;
+  int pre_smooth = 3;
+  int post_smooth = 2;
+  double omega = 0.7;
+  double equil_threshold = 1e-6;  // equilibrate when median |a_ii| is below this
+  int dense_max = 500;            // largest coarsest level solved densely
+  int coarse_sweeps = 50;         // Jacobi sweeps on a coarsest level above dense_max
+  int anchor = -1;                // forced C point on every level (reference anchor_idx), -1 = none
+};
+
+struct LevelInfo {
+  int n, nnz, p_nnz, is_coarsest, dense;
+};
+
+struct Level {
+  int n = 0, nnz = 0, group = 1;
+  bool coarsest = false, dense = false;
+  DevBuf<int> rp, ci;
+  DevBuf<double> v;
+  DevBuf<float> v32;
+  DevBuf<double> ir64;
+  DevBuf<float> ir32;
+  DevBuf<signed char> cf;  // final C/F markers (non-coarsest levels)
+  // interpolation / restriction to the next level
+  int nc = 0, pnnz = 0, pgroup = 1, rgroup = 1;
+  DevBuf<int> prp, pci, rrp, rci;
+  DevBuf<double> pv, rv;
+  DevBuf<float> pv32, rv32;
+  // Galerkin product data: AP = A*P (values refreshed on update), next A = R*AP
+  int apnnz = 0;
+  DevBuf<int> aprp, apci;
+  DevBuf<double> apv;
+  DevBuf<long long> ap_seg, ac_seg;
+  DevBuf<int> ap_pe, ap_pf, ac_pe, ac_pf;
+  // work vectors (x: solution, b: right-hand side, r: residual, t: Jacobi ping-pong)
+  DevBuf<double> x64, b64, r64, t64;
+  DevBuf<float> x32, b32, r32, t32;
+  // coarsest level: [A_pinned^-1] stored in the right half of an n x 2n matrix
+  DevBuf<double> aug;
+};
+
+class Hierarchy {
+ public:
+  explicit Hierarchy(const AmgParams& p) : prm_(p) {}
+
+  // Full setup from an (unscaled) CSR matrix with sorted column indices.
+  void setup_csr(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s);
+  // Setup from a 7-point stencil operator; level 0 then runs index-free.
+  void setup_stencil(int nx, int ny, int nz, const double* d, const double* ox, const double* oy, const double* oz,
+                     hipStream_t s);
+  // New fine-level values, same pattern: re-equilibrate, re-RAP through the
+  // frozen interpolation, refresh relaxation weights and the coarsest inverse
+  // (classical_amg.py:1308-1392).
+  void update_csr_values(const double* v, hipStream_t s);
+  void update_stencil(const double* d, const double* ox, const double* oy, const double* oz, hipStream_t s);
+
+  // One V-cycle in the equilibrated space: x <- V(b) (x_zero: start from 0).
+  void vcycle64(const double* b, double* x, bool x_zero, hipStream_t s);
+  void vcycle32(const float* b, float* x, bool x_zero, hipStream_t s);
+
+  // Preconditioner of the row-scaled system D^-1 A x = D^-1 b:
+  // z = S .* V(S .* D .* v)  with S = D^-1/2 (equilibrated) or I.
+  void precondition(int precision, const double* v, double* z, hipStream_t s);
+
+  int n0() const { return levels_.empty() ? 0 : levels_[0].n; }
+  int num_levels() const { return (int)levels_.size(); }
+  LevelInfo info(int l) const;
+  bool equilibrated() const { return equil_; }
+  const double* dhalf_inv() const { return dh_.p; }  // D^-1/2 or ones
+  const double* diag0() const { return diag0_.p; }   // unscaled fine diagonal
+  const Level& level(int l) const { return levels_[l]; }
+  bool is_stencil() const { return stencil_; }
+  const AmgParams& params() const { return prm_; }
+  double last_setup_seconds = 0.0;
+
+ private:
+  void setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s);
+  void equilibrate(const double* v_unscaled, hipStream_t s);
+  void build_levels(hipStream_t s);
+  void finish_level_values(int l, hipStream_t s);
+  void build_coarsest(int l, hipStream_t s);
+  void refresh_f32(hipStream_t s);
+  void refresh_stencil(hipStream_t s);
+  void alloc_vectors();
+  template <class T>
+  void vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s);
+  template <class T>
+  void smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& zero, hipStream_t s);
+  template <class T>
+  void coarse_solve(int l, const T* b, T* x, hipStream_t s);
+
+  AmgParams prm_;
+  std::vector<Level> levels_;
+  bool equil_ = false;
+  DevBuf<double> v0_unscaled_;  // fine values as given
+  DevBuf<double> diag0_, dh_, sin_, sout_;
+  // stencil form of level 0
+  bool stencil_ = false;
+  int nx_ = 0, ny_ = 0, nz_ = 0;
+  DevBuf<double> st_d_, st_x_, st_y_, st_z_;  // unscaled, as given
+  DevBuf<double> sd_, sx_, sy_, sz_;          // equilibrated
+  DevBuf<float> sd32_, sx32_, sy32_, sz32_;
+  DevBuf<double> pre_b64_, pre_x64_;
+  DevBuf<float> pre_b32_, pre_x32_;
+};
+
+// stencil helpers (amg.hip)
+void launch_st7_pattern_count(int nx, int ny, int nz, int* cnt, hipStream_t s);
+void launch_st7_pattern_fill(int nx, int ny, int nz, const int* rp, int* ci, hipStream_t s);
+void launch_st7_to_csr_values(int nx, int ny, int nz, const double* d, const double* ox, const double* oy,
+                              const double* oz, const int* rp, double* v, hipStream_t s);
+
+}  // namespace impes

@@ -1,0 +1,594 @@
+// Host orchestration of the RS-AMG hierarchy: setup, numeric update and V-cycle.
+#include "amg.h"
+
+#include <chrono>
+
+#include "amg_kernels.h"
+#include "prims.h"
+
+namespace impes {
+
+namespace {
+
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+void d2d(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n) IMPES_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+// ---------------------------------------------------------------- stencil kernels
+__device__ __forceinline__ void cell_ijk(long long c, int ny, int nz, int& i, int& j, int& k) {
+  k = (int)(c % nz);
+  const long long t = c / nz;
+  j = (int)(t % ny);
+  i = (int)(t / ny);
+}
+
+__global__ void k_st7_count(int nx, int ny, int nz, int* __restrict__ cnt) {
+  const long long n = (long long)nx * ny * nz;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(c, ny, nz, i, j, k);
+    cnt[c] = 1 + (i > 0) + (j > 0) + (k > 0) + (k < nz - 1) + (j < ny - 1) + (i < nx - 1);
+  }
+}
+
+__global__ void k_st7_fill(int nx, int ny, int nz, const int* __restrict__ rp, int* __restrict__ ci) {
+  const long long n = (long long)nx * ny * nz;
+  const long long sx = (long long)ny * nz, sy = nz;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(c, ny, nz, i, j, k);
+    int e = rp[c];
+    if (i > 0) ci[e++] = (int)(c - sx);
+    if (j > 0) ci[e++] = (int)(c - sy);
+    if (k > 0) ci[e++] = (int)(c - 1);
+    ci[e++] = (int)c;
+    if (k < nz - 1) ci[e++] = (int)(c + 1);
+    if (j < ny - 1) ci[e++] = (int)(c + sy);
+    if (i < nx - 1) ci[e++] = (int)(c + sx);
+  }
+}
+
+__global__ void k_st7_vals(int nx, int ny, int nz, const double* __restrict__ d, const double* __restrict__ ox,
+                           const double* __restrict__ oy, const double* __restrict__ oz, const int* __restrict__ rp,
+                           double* __restrict__ v) {
+  const long long n = (long long)nx * ny * nz;
+  const long long sx = (long long)ny * nz, sy = nz;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(c, ny, nz, i, j, k);
+    int e = rp[c];
+    if (i > 0) v[e++] = ox[c - sx];
+    if (j > 0) v[e++] = oy[c - sy];
+    if (k > 0) v[e++] = oz[c - 1];
+    v[e++] = d[c];
+    if (k < nz - 1) v[e++] = oz[c];
+    if (j < ny - 1) v[e++] = oy[c];
+    if (i < nx - 1) v[e++] = ox[c];
+  }
+}
+
+// equilibrated stencil (and its FP32 copy): a_ij * dh_i * dh_j
+__global__ void k_st7_scale(int nx, int ny, int nz, const double* __restrict__ d, const double* __restrict__ ox,
+                            const double* __restrict__ oy, const double* __restrict__ oz,
+                            const double* __restrict__ dh, double* __restrict__ sd, double* __restrict__ sxo,
+                            double* __restrict__ syo, double* __restrict__ szo, float* __restrict__ sd32,
+                            float* __restrict__ sx32, float* __restrict__ sy32, float* __restrict__ sz32) {
+  const long long n = (long long)nx * ny * nz;
+  const long long sx = (long long)ny * nz, sy = nz;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    cell_ijk(c, ny, nz, i, j, k);
+    const double h = dh[c];
+    const double vd = d[c] * h * h;
+    const double vx = (i < nx - 1) ? ox[c] * h * dh[c + sx] : 0.0;
+    const double vy = (j < ny - 1) ? oy[c] * h * dh[c + sy] : 0.0;
+    const double vz = (k < nz - 1) ? oz[c] * h * dh[c + 1] : 0.0;
+    sd[c] = vd; sxo[c] = vx; syo[c] = vy; szo[c] = vz;
+    sd32[c] = (float)vd; sx32[c] = (float)vx; sy32[c] = (float)vy; sz32[c] = (float)vz;
+  }
+}
+
+__global__ void k_set_pinned_zero(double* __restrict__ b) { b[0] = 0.0; }
+__global__ void k_set_pinned_zero32(float* __restrict__ b) { b[0] = 0.f; }
+__global__ void k_set_cf(signed char* cf, int idx) { cf[idx] = kC; }
+__global__ void k_neg_flags(int m, const int* __restrict__ tp, int* __restrict__ out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < m; e += (long long)gridDim.x * blockDim.x)
+    out[e] = tp[e] < 0 ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- precision selection
+template <class T>
+struct Sel;
+template <>
+struct Sel<double> {
+  static double* x(Level& L) { return L.x64.p; }
+  static double* b(Level& L) { return L.b64.p; }
+  static double* r(Level& L) { return L.r64.p; }
+  static double* t(Level& L) { return L.t64.p; }
+  static const double* ir(const Level& L) { return L.ir64.p; }
+  static CsrF64 A(const Level& L) { return CsrF64{L.n, L.n, L.nnz, L.rp.p, L.ci.p, L.v.p}; }
+  static CsrF64 P(const Level& L) { return CsrF64{L.n, L.nc, L.pnnz, L.prp.p, L.pci.p, L.pv.p}; }
+  static CsrF64 R(const Level& L) { return CsrF64{L.nc, L.n, L.pnnz, L.rrp.p, L.rci.p, L.rv.p}; }
+  static void jac(const CsrF64& A, const double* xi, const double* b, const double* ir, double w, double* xo, int g,
+                  hipStream_t s) { launch_jacobi_f64(A, xi, b, ir, w, xo, g, s); }
+  static void jac0(const double* b, const double* ir, double w, double* xo, int n, hipStream_t s) {
+    launch_jacobi0_f64(b, ir, w, xo, n, s);
+  }
+  static void res(const CsrF64& A, const double* x, const double* b, double* r, int g, hipStream_t s) {
+    launch_residual_f64(A, x, b, r, g, s);
+  }
+  static void spmv(const CsrF64& A, const double* x, double* y, int g, hipStream_t s) { launch_spmv_f64(A, x, y, g, s); }
+  static void spmv_add(const CsrF64& A, const double* x, double* y, int g, hipStream_t s) {
+    launch_spmv_add_f64(A, x, y, g, s);
+  }
+  static void copy(double* y, const double* x, long long n, hipStream_t s) { launch_copy_f64(y, x, n, s); }
+  static void pin0(double* b, hipStream_t s) { k_set_pinned_zero<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
+  static void dense(int n, const double* aug, const double* b, double* x, hipStream_t s) {
+    launch_dense_solve_f64(n, aug, b, x, s);
+  }
+};
+template <>
+struct Sel<float> {
+  static float* x(Level& L) { return L.x32.p; }
+  static float* b(Level& L) { return L.b32.p; }
+  static float* r(Level& L) { return L.r32.p; }
+  static float* t(Level& L) { return L.t32.p; }
+  static const float* ir(const Level& L) { return L.ir32.p; }
+  static CsrF32 A(const Level& L) { return CsrF32{L.n, L.n, L.nnz, L.rp.p, L.ci.p, L.v32.p}; }
+  static CsrF32 P(const Level& L) { return CsrF32{L.n, L.nc, L.pnnz, L.prp.p, L.pci.p, L.pv32.p}; }
+  static CsrF32 R(const Level& L) { return CsrF32{L.nc, L.n, L.pnnz, L.rrp.p, L.rci.p, L.rv32.p}; }
+  static void jac(const CsrF32& A, const float* xi, const float* b, const float* ir, double w, float* xo, int g,
+                  hipStream_t s) { launch_jacobi_f32(A, xi, b, ir, (float)w, xo, g, s); }
+  static void jac0(const float* b, const float* ir, double w, float* xo, int n, hipStream_t s) {
+    launch_jacobi0_f32(b, ir, (float)w, xo, n, s);
+  }
+  static void res(const CsrF32& A, const float* x, const float* b, float* r, int g, hipStream_t s) {
+    launch_residual_f32(A, x, b, r, g, s);
+  }
+  static void spmv(const CsrF32& A, const float* x, float* y, int g, hipStream_t s) { launch_spmv_f32(A, x, y, g, s); }
+  static void spmv_add(const CsrF32& A, const float* x, float* y, int g, hipStream_t s) {
+    launch_spmv_add_f32(A, x, y, g, s);
+  }
+  static void copy(float* y, const float* x, long long n, hipStream_t s) { launch_copy_f32(y, x, n, s); }
+  static void pin0(float* b, hipStream_t s) { k_set_pinned_zero32<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
+  static void dense(int n, const double* aug, const float* b, float* x, hipStream_t s) {
+    launch_dense_solve_f32(n, aug, b, x, s);
+  }
+};
+
+}  // namespace
+
+void launch_st7_pattern_count(int nx, int ny, int nz, int* cnt, hipStream_t s) {
+  const long long n = (long long)nx * ny * nz;
+  k_st7_count<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(nx, ny, nz, cnt);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_st7_pattern_fill(int nx, int ny, int nz, const int* rp, int* ci, hipStream_t s) {
+  const long long n = (long long)nx * ny * nz;
+  k_st7_fill<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(nx, ny, nz, rp, ci);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_st7_to_csr_values(int nx, int ny, int nz, const double* d, const double* ox, const double* oy,
+                              const double* oz, const int* rp, double* v, hipStream_t s) {
+  const long long n = (long long)nx * ny * nz;
+  k_st7_vals<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(nx, ny, nz, d, ox, oy, oz, rp, v);
+  IMPES_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- setup
+void Hierarchy::setup_csr(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s) {
+  stencil_ = false;
+  setup_impl(n, nnz, rp, ci, v, s);
+}
+
+void Hierarchy::setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s) {
+  IMPES_REQUIRE(n >= 1 && nnz >= 1, "amg setup: empty matrix");
+  const double t0 = wall();
+  levels_.clear();
+  levels_.emplace_back();
+  Level& L0 = levels_[0];
+  L0.n = n;
+  L0.nnz = nnz;
+  L0.rp.alloc(n + 1);
+  L0.ci.alloc(nnz);
+  L0.v.alloc(nnz);
+  d2d(L0.rp.p, rp, n + 1, s);
+  d2d(L0.ci.p, ci, nnz, s);
+  v0_unscaled_.alloc(nnz);
+  d2d(v0_unscaled_.p, v, nnz, s);
+  equilibrate(v0_unscaled_.p, s);
+  build_levels(s);
+  refresh_f32(s);
+  if (stencil_) refresh_stencil(s);
+  alloc_vectors();
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  last_setup_seconds = wall() - t0;
+}
+
+void Hierarchy::setup_stencil(int nx, int ny, int nz, const double* d, const double* ox, const double* oy,
+                              const double* oz, hipStream_t s) {
+  const long long n = (long long)nx * ny * nz;
+  IMPES_REQUIRE(n >= 1 && n < (1LL << 30), "amg setup: bad stencil grid");
+  stencil_ = true;
+  nx_ = nx; ny_ = ny; nz_ = nz;
+  st_d_.alloc(n); st_x_.alloc(n); st_y_.alloc(n); st_z_.alloc(n);
+  d2d(st_d_.p, d, n, s); d2d(st_x_.p, ox, n, s); d2d(st_y_.p, oy, n, s); d2d(st_z_.p, oz, n, s);
+  DevBuf<int> cnt, rp, ci;
+  DevBuf<double> v;
+  cnt.alloc(n);
+  rp.alloc(n + 1);
+  launch_st7_pattern_count(nx, ny, nz, cnt.p, s);
+  scan_i32_to_i32(cnt.p, rp.p, n, s);
+  const int nnz = read_scalar(rp.p + n, s);
+  ci.alloc(nnz);
+  v.alloc(nnz);
+  launch_st7_pattern_fill(nx, ny, nz, rp.p, ci.p, s);
+  launch_st7_to_csr_values(nx, ny, nz, d, ox, oy, oz, rp.p, v.p, s);
+  setup_impl((int)n, nnz, rp.p, ci.p, v.p, s);
+}
+
+void Hierarchy::equilibrate(const double* vu, hipStream_t s) {
+  Level& L0 = levels_[0];
+  const int n = L0.n;
+  diag0_.alloc(n);
+  dh_.alloc(n);
+  sin_.alloc(n);
+  sout_.alloc(n);
+  launch_diag_relax(n, L0.rp.p, L0.ci.p, vu, diag0_.p, nullptr, nullptr, s);
+  DevBuf<int> flags;
+  flags.alloc(n);
+  launch_count_small_diag(n, diag0_.p, prm_.equil_threshold, flags.p, s);
+  // lower median (torch.median semantics) below the threshold  <=>  at least
+  // (n-1)/2 + 1 entries below it
+  const long long small = sum_i32(flags.p, n, s);
+  equil_ = small >= (long long)((n - 1) / 2 + 1);
+  if (equil_) {
+    launch_dhalf_inv(n, diag0_.p, dh_.p, s);
+    launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
+  } else {
+    launch_fill_f64(dh_.p, 1.0, n, s);
+    d2d(L0.v.p, vu, L0.nnz, s);
+  }
+  launch_scale_f64(sin_.p, dh_.p, diag0_.p, n, s);
+  launch_copy_f64(sout_.p, dh_.p, n, s);
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));  // flags freed on return
+}
+
+void Hierarchy::build_levels(hipStream_t s) {
+  for (int lvl = 0; lvl < prm_.max_levels; ++lvl) {
+    const int n = levels_[lvl].n;
+    if (n <= prm_.coarsest_size) {
+      levels_[lvl].coarsest = true;
+      break;
+    }
+    Level& A = levels_[lvl];
+    const int nnz = A.nnz;
+    DevBuf<unsigned char> strong;
+    DevBuf<int> tpos, deg, is_c, f2c, cnt_c, cand, pcnt, oc, colcnt, cursor, rrp32;
+    DevBuf<signed char> is_max, cf0;
+    DevBuf<double> sum_strong, cvals, ov;
+    DevBuf<long long> cand_ptr, rrp64;
+    DevBuf<unsigned long long> keys;
+    strong.alloc(nnz);
+    tpos.alloc(nnz);
+    launch_strength(n, A.rp.p, A.ci.p, A.v.p, prm_.theta, strong.p, s);
+    launch_transpose_pos(n, A.rp.p, A.ci.p, tpos.p, s);
+    {
+      // structural symmetry is required by the symmetrised strength graph
+      DevBuf<int> neg;
+      neg.alloc(nnz);
+      k_neg_flags<<<grid_for(nnz, 256, 1 << 16), 256, 0, s>>>(nnz, tpos.p, neg.p);
+      IMPES_LAUNCH_CHECK();
+      IMPES_REQUIRE(sum_i32(neg.p, nnz, s) == 0,
+                    "amg setup: matrix pattern is not structurally symmetric");
+    }
+    deg.alloc(n);
+    is_max.alloc(n);
+    cf0.alloc(n);
+    launch_cf_split(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
+    if (prm_.anchor >= 0 && prm_.anchor < n) {
+      k_set_cf<<<1, 1, 0, s>>>(cf0.p, prm_.anchor);
+      IMPES_LAUNCH_CHECK();
+    }
+    const long long n_coarse = count_i8(cf0.p, n, kC, s);
+    if ((double)n_coarse >= 0.9 * n) {
+      A.coarsest = true;
+      break;
+    }
+    // orphan promotion and coarse numbering
+    A.cf.alloc(n);
+    is_c.alloc(n);
+    launch_orphans(n, A.rp.p, A.ci.p, strong.p, cf0.p, A.cf.p, is_c.p, s);
+    f2c.alloc(n + 1);
+    scan_i32_to_i32(is_c.p, f2c.p, n, s);
+    const int nc = read_scalar(f2c.p + n, s);
+    IMPES_REQUIRE(nc >= 1, "amg setup: empty coarse level");
+    // interpolation
+    sum_strong.alloc(n);
+    cnt_c.alloc(n);
+    cand.alloc(n);
+    launch_interp_pass_a(n, A.rp.p, A.ci.p, A.v.p, strong.p, A.cf.p, sum_strong.p, cnt_c.p, s);
+    launch_interp_count(n, A.rp.p, A.ci.p, strong.p, A.cf.p, sum_strong.p, cnt_c.p, cand.p, s);
+    cand_ptr.alloc(n + 1);
+    scan_i32_to_i64(cand.p, cand_ptr.p, n, s);
+    const long long ncand = read_scalar(cand_ptr.p + n, s);
+    keys.alloc(ncand);
+    cvals.alloc(ncand);
+    launch_interp_expand(n, A.rp.p, A.ci.p, A.v.p, strong.p, A.cf.p, sum_strong.p, cnt_c.p, f2c.p, cand_ptr.p, keys.p,
+                         cvals.p, s);
+    segmented_sort_u64(keys.p, cand_ptr.p, n, max_segment_length(cand_ptr.p, n, s), s);
+    oc.alloc(ncand);
+    ov.alloc(ncand);
+    pcnt.alloc(n);
+    launch_interp_reduce(n, A.rp.p, A.ci.p, A.cf.p, f2c.p, cand_ptr.p, keys.p, cvals.p, oc.p, ov.p, pcnt.p, s);
+    A.prp.alloc(n + 1);
+    scan_i32_to_i32(pcnt.p, A.prp.p, n, s);
+    A.pnnz = read_scalar(A.prp.p + n, s);
+    A.pci.alloc(A.pnnz);
+    A.pv.alloc(A.pnnz);
+    launch_interp_gather(n, cand_ptr.p, oc.p, ov.p, A.prp.p, A.pci.p, A.pv.p, s);
+    A.nc = nc;
+    // restriction R = P^T (rows sorted by fine index)
+    colcnt.alloc(nc);
+    launch_fill_i32(colcnt.p, 0, nc, s);
+    launch_col_count(A.pnnz, A.pci.p, colcnt.p, s);
+    rrp64.alloc(nc + 1);
+    scan_i32_to_i64(colcnt.p, rrp64.p, nc, s);
+    A.rrp.alloc(nc + 1);
+    scan_i32_to_i32(colcnt.p, A.rrp.p, nc, s);
+    cursor.alloc(nc);
+    launch_fill_i32(cursor.p, 0, nc, s);
+    {
+      DevBuf<unsigned long long> tkeys;
+      tkeys.alloc(A.pnnz);
+      launch_transpose_fill(n, A.prp.p, A.pci.p, A.rrp.p, cursor.p, tkeys.p, s);
+      segmented_sort_u64(tkeys.p, rrp64.p, nc, max_segment_length(rrp64.p, nc, s), s);
+      A.rci.alloc(A.pnnz);
+      A.rv.alloc(A.pnnz);
+      launch_transpose_finish(A.pnnz, tkeys.p, A.pv.p, A.rci.p, A.rv.p, s);
+      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    // Galerkin product: AP = A*P, Ac = R*AP, both through sorted product lists
+    Level nxt;
+    auto spgemm = [&](int nrows, const int* lrp, const int* lci, const int* mrp, const int* mci, DevBuf<int>& crp,
+                      DevBuf<int>& cci, int& cnnz, DevBuf<long long>& seg, DevBuf<int>& pe, DevBuf<int>& pf) {
+      DevBuf<int> pc, uc;
+      DevBuf<long long> pptr;
+      DevBuf<unsigned long long> pk;
+      pc.alloc(nrows);
+      launch_spgemm_count(nrows, lrp, lci, mrp, pc.p, s);
+      pptr.alloc(nrows + 1);
+      scan_i32_to_i64(pc.p, pptr.p, nrows, s);
+      const long long tot = read_scalar(pptr.p + nrows, s);
+      IMPES_REQUIRE(tot < (1LL << 40), "amg setup: product list too large");
+      pk.alloc(tot);
+      launch_spgemm_expand(nrows, lrp, lci, mrp, mci, pptr.p, pk.p, s);
+      segmented_sort_u64(pk.p, pptr.p, nrows, max_segment_length(pptr.p, nrows, s), s);
+      uc.alloc(nrows);
+      launch_spgemm_unique_count(nrows, pptr.p, pk.p, uc.p, s);
+      crp.alloc(nrows + 1);
+      scan_i32_to_i32(uc.p, crp.p, nrows, s);
+      cnnz = read_scalar(crp.p + nrows, s);
+      cci.alloc(cnnz);
+      seg.alloc(cnnz + 1);
+      pe.alloc(tot);
+      pf.alloc(tot);
+      launch_spgemm_finalize(nrows, lrp, lci, mrp, pptr.p, pk.p, crp.p, cci.p, seg.p, pe.p, pf.p, tot, cnnz, s);
+      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    };
+    spgemm(n, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf);
+    A.apv.alloc(A.apnnz);
+    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, s);
+    nxt.n = nc;
+    spgemm(nc, A.rrp.p, A.rci.p, A.aprp.p, A.apci.p, nxt.rp, nxt.ci, nxt.nnz, A.ac_seg, A.ac_pe, A.ac_pf);
+    nxt.v.alloc(nxt.nnz);
+    launch_spgemm_numeric(nxt.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, nxt.v.p, s);
+    A.pgroup = csr_group_for(n, A.pnnz);
+    A.rgroup = csr_group_for(nc, A.pnnz);
+    IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    levels_.push_back(std::move(nxt));
+  }
+  if (!levels_.back().coarsest) levels_.back().coarsest = true;
+  for (int l = 0; l < (int)levels_.size(); ++l) finish_level_values(l, s);
+}
+
+void Hierarchy::finish_level_values(int l, hipStream_t s) {
+  Level& L = levels_[l];
+  L.group = csr_group_for(L.n, L.nnz);
+  L.ir64.alloc(L.n);
+  L.ir32.alloc(L.n);
+  launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
+  if (L.coarsest) build_coarsest(l, s);
+}
+
+void Hierarchy::build_coarsest(int l, hipStream_t s) {
+  Level& L = levels_[l];
+  L.dense = L.n <= prm_.dense_max;
+  if (!L.dense) return;
+  L.aug.alloc(2LL * L.n * L.n);
+  launch_csr_to_dense_pinned(L.n, L.rp.p, L.ci.p, L.v.p, L.aug.p, s);
+  launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+}
+
+void Hierarchy::refresh_f32(hipStream_t s) {
+  for (auto& L : levels_) {
+    L.v32.alloc(L.nnz);
+    launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    if (!L.coarsest) {
+      L.pv32.alloc(L.pnnz);
+      L.rv32.alloc(L.pnnz);
+      launch_f64_to_f32(L.pv32.p, L.pv.p, L.pnnz, s);
+      launch_f64_to_f32(L.rv32.p, L.rv.p, L.pnnz, s);
+    }
+  }
+}
+
+void Hierarchy::refresh_stencil(hipStream_t s) {
+  const long long n = (long long)nx_ * ny_ * nz_;
+  sd_.alloc(n); sx_.alloc(n); sy_.alloc(n); sz_.alloc(n);
+  sd32_.alloc(n); sx32_.alloc(n); sy32_.alloc(n); sz32_.alloc(n);
+  k_st7_scale<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(nx_, ny_, nz_, st_d_.p, st_x_.p, st_y_.p, st_z_.p, dh_.p,
+                                                         sd_.p, sx_.p, sy_.p, sz_.p, sd32_.p, sx32_.p, sy32_.p,
+                                                         sz32_.p);
+  IMPES_LAUNCH_CHECK();
+}
+
+void Hierarchy::alloc_vectors() {
+  for (auto& L : levels_) {
+    L.x64.alloc(L.n); L.b64.alloc(L.n); L.r64.alloc(L.n); L.t64.alloc(L.n);
+    L.x32.alloc(L.n); L.b32.alloc(L.n); L.r32.alloc(L.n); L.t32.alloc(L.n);
+  }
+}
+
+// ---------------------------------------------------------------- update
+void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "amg update before setup");
+  if (v != v0_unscaled_.p) d2d(v0_unscaled_.p, v, levels_[0].nnz, s);
+  equilibrate(v0_unscaled_.p, s);
+  for (int l = 0; l + 1 < (int)levels_.size(); ++l) {
+    Level& A = levels_[l];
+    if (A.coarsest) break;
+    Level& C = levels_[l + 1];
+    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, s);
+    launch_spgemm_numeric(C.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, C.v.p, s);
+  }
+  for (int l = 0; l < (int)levels_.size(); ++l) {
+    Level& L = levels_[l];
+    launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
+    launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    if (L.coarsest && L.dense) {
+      launch_csr_to_dense_pinned(L.n, L.rp.p, L.ci.p, L.v.p, L.aug.p, s);
+      launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+    }
+  }
+  if (stencil_) refresh_stencil(s);
+}
+
+void Hierarchy::update_stencil(const double* d, const double* ox, const double* oy, const double* oz, hipStream_t s) {
+  IMPES_REQUIRE(stencil_, "amg update_stencil on a CSR hierarchy");
+  const long long n = (long long)nx_ * ny_ * nz_;
+  d2d(st_d_.p, d, n, s); d2d(st_x_.p, ox, n, s); d2d(st_y_.p, oy, n, s); d2d(st_z_.p, oz, n, s);
+  Level& L0 = levels_[0];
+  launch_st7_to_csr_values(nx_, ny_, nz_, d, ox, oy, oz, L0.rp.p, v0_unscaled_.p, s);
+  update_csr_values(v0_unscaled_.p, s);
+}
+
+LevelInfo Hierarchy::info(int l) const {
+  const Level& L = levels_.at(l);
+  return LevelInfo{L.n, L.nnz, L.coarsest ? 0 : L.pnnz, L.coarsest ? 1 : 0, L.dense ? 1 : 0};
+}
+
+// ---------------------------------------------------------------- V-cycle
+template <class T>
+void Hierarchy::smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& zero, hipStream_t s) {
+  Level& L = levels_[l];
+  const bool st = (l == 0 && stencil_);
+  for (int it = 0; it < sweeps; ++it) {
+    if (zero) {
+      Sel<T>::jac0(b, Sel<T>::ir(L), prm_.omega, cur, L.n, s);
+      zero = false;
+      continue;
+    }
+    if (st) {
+      if constexpr (sizeof(T) == 8) {
+        Stencil7<double> S{nx_, ny_, nz_, sd_.p, sx_.p, sy_.p, sz_.p};
+        launch_st7_jacobi_f64(S, (const double*)cur, (const double*)b, L.ir64.p, prm_.omega, (double*)other, s);
+      } else {
+        Stencil7<float> S{nx_, ny_, nz_, sd32_.p, sx32_.p, sy32_.p, sz32_.p};
+        launch_st7_jacobi_f32(S, (const float*)cur, (const float*)b, L.ir32.p, (float)prm_.omega, (float*)other, s);
+      }
+    } else {
+      Sel<T>::jac(Sel<T>::A(L), cur, b, Sel<T>::ir(L), prm_.omega, other, L.group, s);
+    }
+    std::swap(cur, other);
+  }
+}
+
+template <class T>
+void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
+  Level& L = levels_[l];
+  if (L.dense) {
+    Sel<T>::dense(L.n, L.aug.p, b, x, s);
+    return;
+  }
+  // reference: 50 L1-Jacobi sweeps from zero on the unpinned matrix with b_0 = 0
+  T* bl = Sel<T>::r(L);
+  Sel<T>::copy(bl, b, L.n, s);
+  Sel<T>::pin0(bl, s);
+  T* cur = x;
+  T* other = Sel<T>::t(L);
+  bool zero = true;
+  smooth<T>(l, bl, cur, other, prm_.coarse_sweeps, zero, s);
+  if (cur != x) Sel<T>::copy(x, cur, L.n, s);
+}
+
+template <class T>
+void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s) {
+  Level& L = levels_[l];
+  if (L.coarsest) {
+    coarse_solve<T>(l, b, x, s);
+    return;
+  }
+  T* cur = x;
+  T* other = Sel<T>::t(L);
+  bool zero = x_zero;
+  smooth<T>(l, b, cur, other, prm_.pre_smooth, zero, s);
+  T* r = Sel<T>::r(L);
+  if (zero) {
+    Sel<T>::copy(r, b, L.n, s);  // no pre-smoothing: r = b - A*0
+  } else if (l == 0 && stencil_) {
+    if constexpr (sizeof(T) == 8) {
+      Stencil7<double> S{nx_, ny_, nz_, sd_.p, sx_.p, sy_.p, sz_.p};
+      launch_st7_residual_f64(S, (const double*)cur, (const double*)b, (double*)r, s);
+    } else {
+      Stencil7<float> S{nx_, ny_, nz_, sd32_.p, sx32_.p, sy32_.p, sz32_.p};
+      launch_st7_residual_f32(S, (const float*)cur, (const float*)b, (float*)r, s);
+    }
+  } else {
+    Sel<T>::res(Sel<T>::A(L), cur, b, r, L.group, s);
+  }
+  Level& C = levels_[l + 1];
+  Sel<T>::spmv(Sel<T>::R(L), r, Sel<T>::b(C), L.rgroup, s);
+  vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s);
+  if (zero) {
+    // no pre-smoothing happened: x = P e_c
+    Sel<T>::spmv(Sel<T>::P(L), Sel<T>::x(C), cur, L.pgroup, s);
+    zero = false;
+  } else {
+    Sel<T>::spmv_add(Sel<T>::P(L), Sel<T>::x(C), cur, L.pgroup, s);
+  }
+  other = (cur == x) ? Sel<T>::t(L) : x;
+  smooth<T>(l, b, cur, other, prm_.post_smooth, zero, s);
+  if (cur != x) Sel<T>::copy(x, cur, L.n, s);
+}
+
+void Hierarchy::vcycle64(const double* b, double* x, bool x_zero, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "vcycle before setup");
+  vcycle_level<double>(0, b, x, x_zero, s);
+}
+
+void Hierarchy::vcycle32(const float* b, float* x, bool x_zero, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "vcycle before setup");
+  vcycle_level<float>(0, b, x, x_zero, s);
+}
+
+void Hierarchy::precondition(int precision, const double* v, double* z, hipStream_t s) {
+  Level& L0 = levels_[0];
+  const long long n = L0.n;
+  if (precision == 32) {
+    launch_scale_f64_to_f32(L0.b32.p, sin_.p, v, n, s);
+    vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
+    launch_scale_f32_to_f64(z, sout_.p, L0.x32.p, n, s);
+  } else {
+    launch_scale_f64(L0.b64.p, sin_.p, v, n, s);
+    vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
+    launch_scale_f64(z, sout_.p, L0.x64.p, n, s);
+  }
+}
+
+}  // namespace impes

@@ -1,0 +1,78 @@
+// Launchers of the Ruge-Stueben setup kernels (amg_setup.hip) and of the dense
+// coarsest-level factor/solve (coarse.hip).
+#pragma once
+
+#include "common.h"
+
+namespace impes {
+
+constexpr signed char kC = 0;  // coarse point marker (reference: cf_marker 0 = C)
+constexpr signed char kF = 1;  // fine point marker
+
+// diag[i] = a_ii (0 if absent); inv_relax = min(1/max(|a_ii|, 0.3*sum_j|a_ij|, 1e-30), 1e2)
+void launch_diag_relax(int n, const int* rp, const int* ci, const double* v, double* diag, double* inv_relax64,
+                       float* inv_relax32, hipStream_t s);
+// number of rows with |a_ii| < thr (lower-median test of the equilibration switch)
+void launch_count_small_diag(int n, const double* diag, double thr, int* flags, hipStream_t s);
+// dh[i] = 1/sqrt(max(|diag_i|, 1e-30))
+void launch_dhalf_inv(int n, const double* diag, double* dh, hipStream_t s);
+// vo[e] = v[e]*dh[row]*dh[col]
+void launch_sym_scale(int n, const int* rp, const int* ci, const double* v, const double* dh, double* vo,
+                      hipStream_t s);
+
+void launch_strength(int n, const int* rp, const int* ci, const double* v, double theta, unsigned char* strong,
+                     hipStream_t s);
+// tpos[e] = position of (col,row) in row col, -1 if absent
+void launch_transpose_pos(int n, const int* rp, const int* ci, int* tpos, hipStream_t s);
+// C/F split: symmetrised strong graph, strict local maxima of (deg, hash, index), lift of F points with no C neighbour
+void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos, int* deg,
+                     signed char* is_max, signed char* cf, hipStream_t s);
+// promote F points with no strong C neighbour
+void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
+                    signed char* cf_out, int* is_c, hipStream_t s);
+
+// interpolation
+void launch_interp_pass_a(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
+                          const signed char* cf, double* sum_strong, int* cnt_c, hipStream_t s);
+void launch_interp_count(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf,
+                         const double* sum_strong, const int* cnt_c, int* cand, hipStream_t s);
+void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
+                          const signed char* cf, const double* sum_strong, const int* cnt_c, const int* f2c,
+                          const long long* cand_ptr, unsigned long long* keys, double* vals, hipStream_t s);
+void launch_interp_reduce(int n, const int* rp, const int* ci, const signed char* cf, const int* f2c,
+                          const long long* cand_ptr, const unsigned long long* keys, const double* vals,
+                          int* out_cols, double* out_vals, int* pcnt, hipStream_t s);
+void launch_interp_gather(int n, const long long* cand_ptr, const int* cols_tmp, const double* vals_tmp,
+                          const int* prp, int* pci, double* pv, hipStream_t s);
+
+// transpose (R = P^T): counts per column, fill, key build, gather after sort
+void launch_col_count(int nnz, const int* ci, int* cnt, hipStream_t s);
+void launch_transpose_fill(int n_rows, const int* rp, const int* ci, const int* rrp, int* cursor,
+                           unsigned long long* keys, hipStream_t s);
+// after the per-column sort: rci[p] = source row, rv[p] = source value
+void launch_transpose_finish(int nnz, const unsigned long long* keys, const double* src_v, int* rci, double* rv,
+                             hipStream_t s);
+
+// SpGEMM C = L*M, symbolic through expanded products sorted per row
+void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, int* cnt, hipStream_t s);
+void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
+                          const long long* prod_ptr, unsigned long long* keys, hipStream_t s);
+void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
+                                hipStream_t s);
+// writes C column indices, product segment pointers (int64, nnzC+1) and the sorted
+// operand index pairs (left entry, right entry) of every product
+void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mrp, const long long* prod_ptr,
+                            const unsigned long long* keys, const int* crp, int* cci, long long* seg, int* pe,
+                            int* pf, long long total_products, int nnz_c, hipStream_t s);
+// C.v[o] = sum_{t in seg(o)} Lv[pe[t]] * Mv[pf[t]]
+void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
+                           const double* mv, double* cv, hipStream_t s);
+
+// coarsest level (coarse.hip): dense inverse of the DOF-0-pinned matrix
+void launch_csr_to_dense_pinned(int n, const int* rp, const int* ci, const double* v, double* dense, hipStream_t s);
+void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s);  // aug is n x 2n row-major, inverse in right half
+// x = Ainv * b with b[0] treated as 0 (pinned DOF); FP64 and FP32 vector variants
+void launch_dense_solve_f64(int n, const double* aug, const double* b, double* x, hipStream_t s);
+void launch_dense_solve_f32(int n, const double* aug, const float* b, float* x, hipStream_t s);
+
+}  // namespace impes

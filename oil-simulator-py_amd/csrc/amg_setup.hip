@@ -1,0 +1,562 @@
+// Ruge-Stueben AMG setup kernels.
+//
+// Restates, per row and without host round trips, the setup of
+// src/solver/classical_amg.py:
+//   strength of connection            find_strong_connections   :34-82
+//   C/F split                         classical_coarsening      :85-146
+//   orphan promotion + interpolation  build_prolongation_rs_full :149-427
+//   Galerkin product                  rap_torch / _csr_spmm     :519-657
+//   L1-Jacobi relaxation weights      ClassicalAMG.__init__     :885-889
+// The unseeded torch.rand tie-break of the C/F split (:125) is replaced by a
+// deterministic integer hash of the row index, so the hierarchy is reproducible.
+#include "amg_kernels.h"
+
+namespace impes {
+
+namespace {
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ unsigned hash32(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+// lexicographic (deg, hash, index) comparison: true if node a ranks above node b
+__device__ __forceinline__ bool ranks_above(int deg_a, int a, int deg_b, int b) {
+  if (deg_a != deg_b) return deg_a > deg_b;
+  const unsigned ha = hash32((unsigned)a), hb = hash32((unsigned)b);
+  if (ha != hb) return ha > hb;
+  return a > b;
+}
+
+#define ROW_LOOP(i, n) for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+__global__ void k_diag_relax(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const double* __restrict__ v, double* __restrict__ diag, double* __restrict__ ir64,
+                             float* __restrict__ ir32) {
+  ROW_LOOP(i, n) {
+    double d = 0.0, rs = 0.0;
+    bool found = false;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      rs += fabs(v[e]);
+      if (ci[e] == i && !found) { d = v[e]; found = true; }
+    }
+    if (diag) diag[i] = d;
+    double den = fmax(fabs(d), 0.3 * rs);
+    den = fmax(den, 1e-30);
+    const double ir = fmin(1.0 / den, 1e2);
+    if (ir64) ir64[i] = ir;
+    if (ir32) ir32[i] = (float)ir;
+  }
+}
+
+__global__ void k_small_diag(int n, const double* __restrict__ diag, double thr, int* __restrict__ flags) {
+  ROW_LOOP(i, n) flags[i] = (fmax(fabs(diag[i]), 1e-30) < thr) ? 1 : 0;
+}
+
+__global__ void k_dhalf_inv(int n, const double* __restrict__ diag, double* __restrict__ dh) {
+  ROW_LOOP(i, n) dh[i] = 1.0 / sqrt(fmax(fabs(diag[i]), 1e-30));
+}
+
+__global__ void k_sym_scale(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ v, const double* __restrict__ dh, double* __restrict__ vo) {
+  ROW_LOOP(i, n) {
+    const double di = dh[i];
+    for (int e = rp[i]; e < rp[i + 1]; ++e) vo[e] = v[e] * di * dh[ci[e]];
+  }
+}
+
+__global__ void k_strength(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                           double theta, unsigned char* __restrict__ strong) {
+  ROW_LOOP(i, n) {
+    double d = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+      if (ci[e] == i) { d = fabs(v[e]); break; }
+    const double den = d + 1e-30;
+    double mx = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const double sv = v[e] / den;
+      if (ci[e] != i && sv < 0.0) mx = fmax(mx, -sv);
+    }
+    const double thr = theta * mx;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const double sv = v[e] / den;
+      strong[e] = (ci[e] != i && sv < 0.0 && -sv >= thr) ? 1 : 0;
+    }
+  }
+}
+
+__global__ void k_transpose_pos(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* __restrict__ tpos) {
+  ROW_LOOP(i, n) {
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      int lo = rp[j], hi = rp[j + 1] - 1, pos = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = ci[mid];
+        if (c == (int)i) { pos = mid; break; }
+        if (c < (int)i) lo = mid + 1; else hi = mid - 1;
+      }
+      tpos[e] = pos;
+    }
+  }
+}
+
+__device__ __forceinline__ bool in_graph(const unsigned char* strong, const int* tpos, int e) {
+  const int t = tpos[e];
+  return strong[e] || (t >= 0 && strong[t]);
+}
+
+__global__ void k_cf_deg(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const unsigned char* __restrict__ strong, const int* __restrict__ tpos, int* __restrict__ deg) {
+  ROW_LOOP(i, n) {
+    int d = 0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+      if (ci[e] != (int)i && in_graph(strong, tpos, e)) ++d;
+    deg[i] = d;
+  }
+}
+
+__global__ void k_cf_select(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                            const int* __restrict__ deg, signed char* __restrict__ is_max) {
+  ROW_LOOP(i, n) {
+    const int di = deg[i];
+    bool mx = true;
+    for (int e = rp[i]; e < rp[i + 1] && mx; ++e) {
+      const int j = ci[e];
+      if (j != (int)i && in_graph(strong, tpos, e) && ranks_above(deg[j], j, di, (int)i)) mx = false;
+    }
+    is_max[i] = mx ? 1 : 0;
+  }
+}
+
+__global__ void k_cf_lift(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                          const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                          const signed char* __restrict__ is_max, signed char* __restrict__ cf) {
+  ROW_LOOP(i, n) {
+    if (is_max[i]) { cf[i] = kC; continue; }
+    bool has_c = false;
+    for (int e = rp[i]; e < rp[i + 1] && !has_c; ++e) {
+      const int j = ci[e];
+      if (j != (int)i && in_graph(strong, tpos, e) && is_max[j]) has_c = true;
+    }
+    cf[i] = has_c ? kF : kC;
+  }
+}
+
+__global__ void k_orphans(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                          const unsigned char* __restrict__ strong, const signed char* __restrict__ cf_in,
+                          signed char* __restrict__ cf_out, int* __restrict__ is_c) {
+  ROW_LOOP(i, n) {
+    signed char c = cf_in[i];
+    if (c == kF) {
+      bool has = false;
+      for (int e = rp[i]; e < rp[i + 1] && !has; ++e) {
+        const int j = ci[e];
+        if (j != (int)i && strong[e] && cf_in[j] == kC) has = true;
+      }
+      if (!has) c = kC;
+    }
+    cf_out[i] = c;
+    is_c[i] = (c == kC) ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------- interpolation
+__global__ void k_interp_a(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+                           const unsigned char* __restrict__ strong, const signed char* __restrict__ cf,
+                           double* __restrict__ sum_strong, int* __restrict__ cnt_c) {
+  ROW_LOOP(i, n) {
+    double s = 0.0;
+    int c = 0;
+    if (cf[i] == kF) {
+      for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (j != (int)i && strong[e] && cf[j] == kC) { s += -v[e]; ++c; }
+      }
+    }
+    sum_strong[i] = s;
+    cnt_c[i] = c;
+  }
+}
+
+__device__ __forceinline__ bool ff_valid(const double* sum_strong, const int* cnt_c, int j) {
+  return cnt_c[j] > 0 && sum_strong[j] > 1e-14;
+}
+
+__global__ void k_interp_count(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const unsigned char* __restrict__ strong, const signed char* __restrict__ cf,
+                               const double* __restrict__ sum_strong, const int* __restrict__ cnt_c,
+                               int* __restrict__ cand) {
+  ROW_LOOP(i, n) {
+    if (cf[i] == kC) { cand[i] = 1; continue; }
+    int c = cnt_c[i];
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      if (j != (int)i && strong[e] && cf[j] == kF && ff_valid(sum_strong, cnt_c, j)) c += cnt_c[j];
+    }
+    // one spare slot (sentinel key) for the zero-row fallback insertion
+    cand[i] = c + 1;
+  }
+}
+
+__device__ __forceinline__ double finite_or_zero(double x) { return isfinite(x) ? x : 0.0; }
+
+__global__ void k_interp_expand(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ v, const unsigned char* __restrict__ strong,
+                                const signed char* __restrict__ cf, const double* __restrict__ sum_strong,
+                                const int* __restrict__ cnt_c, const int* __restrict__ f2c,
+                                const long long* __restrict__ cand_ptr, unsigned long long* __restrict__ keys,
+                                double* __restrict__ vals) {
+  ROW_LOOP(i, n) {
+    const long long base = cand_ptr[i];
+    if (cf[i] == kC) {
+      keys[base] = ((unsigned long long)(unsigned)f2c[i]) << 32;
+      vals[base] = 1.0;
+      continue;
+    }
+    double diag = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+      if (ci[e] == (int)i) { diag += v[e]; }
+    if (!(fabs(diag) > 1e-14)) diag = 1.0;
+    double weak_c = 0.0, f_weak = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      if (j == (int)i) continue;
+      if (!strong[e]) {
+        if (cf[j] == kC) weak_c += -v[e];
+        else f_weak += -v[e];
+      } else if (cf[j] == kF && !ff_valid(sum_strong, cnt_c, j)) {
+        f_weak += -v[e];
+      }
+    }
+    const double ss = sum_strong[i];
+    const double corr = (ss > 1e-14) ? (weak_c + f_weak) / ss : 0.0;
+    unsigned local = 0;
+    // direct strong C neighbours
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      if (j != (int)i && strong[e] && cf[j] == kC) {
+        double w = finite_or_zero(-v[e] / diag) * (1.0 + corr);
+        w = fmax(w, 0.0);
+        keys[base + local] = (((unsigned long long)(unsigned)f2c[j]) << 32) | local;
+        vals[base + local] = w;
+        ++local;
+      }
+    }
+    // distance-two contributions through strong F neighbours
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      if (j == (int)i || !strong[e] || cf[j] != kF || !ff_valid(sum_strong, cnt_c, j)) continue;
+      const double factor = finite_or_zero(-v[e] / diag);
+      const double ssj = sum_strong[j];
+      for (int e2 = rp[j]; e2 < rp[j + 1]; ++e2) {
+        const int k = ci[e2];
+        if (k != j && strong[e2] && cf[k] == kC) {
+          const double jw = -v[e2] / ssj;
+          const double c = fmax(factor * jw, 0.0);
+          keys[base + local] = (((unsigned long long)(unsigned)f2c[k]) << 32) | local;
+          vals[base + local] = c;
+          ++local;
+        }
+      }
+    }
+    keys[base + local] = ~0ull;  // spare slot, sorts last
+    vals[base + local] = 0.0;
+  }
+}
+
+__global__ void k_interp_reduce(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const signed char* __restrict__ cf, const int* __restrict__ f2c,
+                                const long long* __restrict__ cand_ptr, const unsigned long long* __restrict__ keys,
+                                const double* __restrict__ vals, int* __restrict__ oc, double* __restrict__ ov,
+                                int* __restrict__ pcnt) {
+  ROW_LOOP(i, n) {
+    const long long b = cand_ptr[i], e_end = cand_ptr[i + 1];
+    if (cf[i] == kC) {
+      oc[b] = f2c[i];
+      ov[b] = 1.0;
+      pcnt[i] = 1;
+      continue;
+    }
+    // merge duplicates (segment sorted by coarse column, ties in generation order)
+    int u = 0;
+    double row_sum = 0.0;
+    for (long long t = b; t < e_end && keys[t] != ~0ull;) {
+      const int col = (int)(keys[t] >> 32);
+      double s = 0.0;
+      while (t < e_end && keys[t] != ~0ull && (int)(keys[t] >> 32) == col) {
+        s += vals[b + (long long)(keys[t] & 0xffffffffull)];
+        ++t;
+      }
+      oc[b + u] = col;
+      ov[b + u] = s;
+      row_sum += s;
+      ++u;
+    }
+    double scale = 1.0;
+    if (row_sum > 1e-14) {
+      scale = 1.0 / row_sum;
+    } else {
+      // no usable weight: inject to the first coarse neighbour of the row, keeping
+      // whatever (tiny) weights exist, then renormalise (classical_amg.py:341-384)
+      int target = -1;
+      for (int e = rp[i]; e < rp[i + 1]; ++e)
+        if (cf[ci[e]] == kC) { target = f2c[ci[e]]; break; }
+      if (target >= 0) {
+        // insert (target, +1) keeping the row sorted
+        int pos = 0;
+        while (pos < u && oc[b + pos] < target) ++pos;
+        if (pos < u && oc[b + pos] == target) {
+          ov[b + pos] += 1.0;
+        } else {
+          // the segment of an F row has one spare slot beyond its candidates, so
+          // u + 1 <= segment length here
+          for (int q = u; q > pos; --q) { oc[b + q] = oc[b + q - 1]; ov[b + q] = ov[b + q - 1]; }
+          oc[b + pos] = target;
+          ov[b + pos] = 1.0;
+          ++u;
+        }
+        double rs = 0.0;
+        for (int q = 0; q < u; ++q) rs += ov[b + q];
+        scale = (rs > 1e-14) ? 1.0 / rs : 1.0;
+      }
+    }
+    int w = 0;
+    for (int q = 0; q < u; ++q) {
+      const double val = fmax(ov[b + q] * scale, 0.0);
+      if (val != 0.0) {
+        oc[b + w] = oc[b + q];
+        ov[b + w] = val;
+        ++w;
+      }
+    }
+    pcnt[i] = w;
+  }
+}
+
+__global__ void k_interp_gather(int n, const long long* __restrict__ cand_ptr, const int* __restrict__ oc,
+                                const double* __restrict__ ov, const int* __restrict__ prp, int* __restrict__ pci,
+                                double* __restrict__ pv) {
+  ROW_LOOP(i, n) {
+    const long long b = cand_ptr[i];
+    const int o = prp[i], cnt = prp[i + 1] - prp[i];
+    for (int q = 0; q < cnt; ++q) {
+      pci[o + q] = oc[b + q];
+      pv[o + q] = ov[b + q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- transpose
+__global__ void k_col_count(int nnz, const int* __restrict__ ci, int* __restrict__ cnt) {
+  ROW_LOOP(e, nnz) atomicAdd(&cnt[ci[e]], 1);
+}
+
+__global__ void k_transpose_fill(int n_rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const int* __restrict__ rrp, int* __restrict__ cursor,
+                                 unsigned long long* __restrict__ keys) {
+  ROW_LOOP(i, n_rows) {
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int c = ci[e];
+      const int slot = atomicAdd(&cursor[c], 1);
+      keys[rrp[c] + slot] = (((unsigned long long)(unsigned)i) << 32) | (unsigned)e;
+    }
+  }
+}
+
+__global__ void k_transpose_finish(int nnz, const unsigned long long* __restrict__ keys,
+                                   const double* __restrict__ src_v, int* __restrict__ rci, double* __restrict__ rv) {
+  ROW_LOOP(p, nnz) {
+    const unsigned long long k = keys[p];
+    rci[p] = (int)(k >> 32);
+    rv[p] = src_v[k & 0xffffffffull];
+  }
+}
+
+// ---------------------------------------------------------------- SpGEMM
+__global__ void k_spgemm_count(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
+                               const int* __restrict__ mrp, int* __restrict__ cnt) {
+  ROW_LOOP(i, n) {
+    int c = 0;
+    for (int e = lrp[i]; e < lrp[i + 1]; ++e) {
+      const int k = lci[e];
+      c += mrp[k + 1] - mrp[k];
+    }
+    cnt[i] = c;
+  }
+}
+
+__global__ void k_spgemm_expand(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
+                                const int* __restrict__ mrp, const int* __restrict__ mci,
+                                const long long* __restrict__ prod_ptr, unsigned long long* __restrict__ keys) {
+  ROW_LOOP(i, n) {
+    long long t = prod_ptr[i];
+    unsigned local = 0;
+    for (int e = lrp[i]; e < lrp[i + 1]; ++e) {
+      const int k = lci[e];
+      for (int f = mrp[k]; f < mrp[k + 1]; ++f) {
+        keys[t++] = (((unsigned long long)(unsigned)mci[f]) << 32) | local;
+        ++local;
+      }
+    }
+  }
+}
+
+__global__ void k_spgemm_unique(int n, const long long* __restrict__ prod_ptr,
+                                const unsigned long long* __restrict__ keys, int* __restrict__ cnt) {
+  ROW_LOOP(i, n) {
+    int c = 0;
+    int prev = -1;
+    for (long long t = prod_ptr[i]; t < prod_ptr[i + 1]; ++t) {
+      const int col = (int)(keys[t] >> 32);
+      if (col != prev) { ++c; prev = col; }
+    }
+    cnt[i] = c;
+  }
+}
+
+// local product index -> (left entry, right entry) by replaying the row's expansion order
+__global__ void k_spgemm_finalize(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
+                                  const int* __restrict__ mrp, const long long* __restrict__ prod_ptr,
+                                  const unsigned long long* __restrict__ keys, const int* __restrict__ crp,
+                                  int* __restrict__ cci, long long* __restrict__ seg, int* __restrict__ pe,
+                                  int* __restrict__ pf, long long total, int nnz_c) {
+  ROW_LOOP(i, n) {
+    const long long b = prod_ptr[i], e_end = prod_ptr[i + 1];
+    // the local index l enumerates (e over L row i, f over M row lci[e]) in order;
+    // map l -> (e,f) through the running offsets of L row i.
+    const int le0 = lrp[i], le1 = lrp[i + 1];
+    int o = crp[i];
+    int prev = -1;
+    for (long long t = b; t < e_end; ++t) {
+      const unsigned long long key = keys[t];
+      const int col = (int)(key >> 32);
+      unsigned l = (unsigned)(key & 0xffffffffull);
+      int e = le0;
+      // locate the left entry whose block of products holds l
+      for (; e < le1; ++e) {
+        const int k = lci[e];
+        const unsigned len = (unsigned)(mrp[k + 1] - mrp[k]);
+        if (l < len) break;
+        l -= len;
+      }
+      pe[t] = e;
+      pf[t] = mrp[lci[e]] + (int)l;
+      if (col != prev) {
+        cci[o] = col;
+        seg[o] = t;
+        ++o;
+        prev = col;
+      }
+    }
+    if (i == n - 1) seg[nnz_c] = total;
+  }
+}
+
+__global__ void k_spgemm_numeric(int nnz_c, const long long* __restrict__ seg, const int* __restrict__ pe,
+                                 const int* __restrict__ pf, const double* __restrict__ lv,
+                                 const double* __restrict__ mv, double* __restrict__ cv) {
+  ROW_LOOP(o, nnz_c) {
+    double s = 0.0;
+    for (long long t = seg[o]; t < seg[o + 1]; ++t) s += lv[pe[t]] * mv[pf[t]];
+    cv[o] = s;
+  }
+}
+
+template <class K, class... A>
+void launch_rows(K kernel, long long n, hipStream_t s, A... args) {
+  if (n <= 0) return;
+  kernel<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(args...);
+  IMPES_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void launch_diag_relax(int n, const int* rp, const int* ci, const double* v, double* diag, double* ir64, float* ir32,
+                       hipStream_t s) {
+  launch_rows(k_diag_relax, n, s, n, rp, ci, v, diag, ir64, ir32);
+}
+void launch_count_small_diag(int n, const double* diag, double thr, int* flags, hipStream_t s) {
+  launch_rows(k_small_diag, n, s, n, diag, thr, flags);
+}
+void launch_dhalf_inv(int n, const double* diag, double* dh, hipStream_t s) { launch_rows(k_dhalf_inv, n, s, n, diag, dh); }
+void launch_sym_scale(int n, const int* rp, const int* ci, const double* v, const double* dh, double* vo,
+                      hipStream_t s) {
+  launch_rows(k_sym_scale, n, s, n, rp, ci, v, dh, vo);
+}
+void launch_strength(int n, const int* rp, const int* ci, const double* v, double theta, unsigned char* strong,
+                     hipStream_t s) {
+  launch_rows(k_strength, n, s, n, rp, ci, v, theta, strong);
+}
+void launch_transpose_pos(int n, const int* rp, const int* ci, int* tpos, hipStream_t s) {
+  launch_rows(k_transpose_pos, n, s, n, rp, ci, tpos);
+}
+void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos, int* deg,
+                     signed char* is_max, signed char* cf, hipStream_t s) {
+  launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
+  launch_rows(k_cf_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, is_max);
+  launch_rows(k_cf_lift, n, s, n, rp, ci, strong, tpos, (const signed char*)is_max, cf);
+}
+void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
+                    signed char* cf_out, int* is_c, hipStream_t s) {
+  launch_rows(k_orphans, n, s, n, rp, ci, strong, cf_in, cf_out, is_c);
+}
+void launch_interp_pass_a(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
+                          const signed char* cf, double* sum_strong, int* cnt_c, hipStream_t s) {
+  launch_rows(k_interp_a, n, s, n, rp, ci, v, strong, cf, sum_strong, cnt_c);
+}
+void launch_interp_count(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf,
+                         const double* sum_strong, const int* cnt_c, int* cand, hipStream_t s) {
+  launch_rows(k_interp_count, n, s, n, rp, ci, strong, cf, sum_strong, cnt_c, cand);
+}
+void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
+                          const signed char* cf, const double* sum_strong, const int* cnt_c, const int* f2c,
+                          const long long* cand_ptr, unsigned long long* keys, double* vals, hipStream_t s) {
+  launch_rows(k_interp_expand, n, s, n, rp, ci, v, strong, cf, sum_strong, cnt_c, f2c, cand_ptr, keys, vals);
+}
+void launch_interp_reduce(int n, const int* rp, const int* ci, const signed char* cf, const int* f2c,
+                          const long long* cand_ptr, const unsigned long long* keys, const double* vals,
+                          int* out_cols, double* out_vals, int* pcnt, hipStream_t s) {
+  launch_rows(k_interp_reduce, n, s, n, rp, ci, cf, f2c, cand_ptr, keys, vals, out_cols, out_vals, pcnt);
+}
+void launch_interp_gather(int n, const long long* cand_ptr, const int* cols_tmp, const double* vals_tmp,
+                          const int* prp, int* pci, double* pv, hipStream_t s) {
+  launch_rows(k_interp_gather, n, s, n, cand_ptr, cols_tmp, vals_tmp, prp, pci, pv);
+}
+void launch_col_count(int nnz, const int* ci, int* cnt, hipStream_t s) { launch_rows(k_col_count, nnz, s, nnz, ci, cnt); }
+void launch_transpose_fill(int n_rows, const int* rp, const int* ci, const int* rrp, int* cursor,
+                           unsigned long long* keys, hipStream_t s) {
+  launch_rows(k_transpose_fill, n_rows, s, n_rows, rp, ci, rrp, cursor, keys);
+}
+void launch_transpose_finish(int nnz, const unsigned long long* keys, const double* src_v, int* rci, double* rv,
+                             hipStream_t s) {
+  launch_rows(k_transpose_finish, nnz, s, nnz, keys, src_v, rci, rv);
+}
+void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, int* cnt, hipStream_t s) {
+  launch_rows(k_spgemm_count, n, s, n, lrp, lci, mrp, cnt);
+}
+void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
+                          const long long* prod_ptr, unsigned long long* keys, hipStream_t s) {
+  launch_rows(k_spgemm_expand, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys);
+}
+void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
+                                hipStream_t s) {
+  launch_rows(k_spgemm_unique, n, s, n, prod_ptr, keys, cnt);
+}
+void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mrp, const long long* prod_ptr,
+                            const unsigned long long* keys, const int* crp, int* cci, long long* seg, int* pe,
+                            int* pf, long long total_products, int nnz_c, hipStream_t s) {
+  launch_rows(k_spgemm_finalize, n, s, n, lrp, lci, mrp, prod_ptr, keys, crp, cci, seg, pe, pf, total_products, nnz_c);
+}
+void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
+                           const double* mv, double* cv, hipStream_t s) {
+  launch_rows(k_spgemm_numeric, nnz_c, s, nnz_c, seg, pe, pf, lv, mv, cv);
+}
+
+}  // namespace impes

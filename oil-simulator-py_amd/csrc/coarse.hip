@@ -1,0 +1,136 @@
+// Coarsest-level direct solve of the V-cycle (classical_amg.py:1045-1069):
+// the level matrix is made dense, DOF 0 is pinned (row/column 0 zeroed, a_00 = 1,
+// b_0 = 0), and the system is solved exactly. The inverse is formed once per
+// hierarchy update by Gauss-Jordan elimination with partial pivoting in FP64, so
+// each V-cycle visit is a single dense matrix-vector product.
+#include "amg_kernels.h"
+
+namespace impes {
+
+namespace {
+
+__global__ void k_csr_to_dense_pinned(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                      const double* __restrict__ v, double* __restrict__ aug) {
+  // aug is n x 2n: [A_pinned | I]
+  const long long w = 2LL * n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < (long long)n * w;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(t / w), c = (int)(t % w);
+    aug[t] = (c >= n && c - n == r) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void k_dense_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ v, double* __restrict__ aug) {
+  const long long w = 2LL * n;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      const int c = ci[e];
+      if (r == 0 || c == 0) continue;  // pinned DOF 0
+      aug[r * w + c] += v[e];
+    }
+    if (r == 0) aug[0] = 1.0;
+  }
+}
+
+// Single workgroup Gauss-Jordan on the n x 2n augmented matrix (n <= 500).
+__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ aug) {
+  __shared__ double fcol[512];
+  __shared__ double red_v[1024 / kWave];
+  __shared__ int red_i[1024 / kWave];
+  __shared__ int piv_row;
+  const long long w = 2LL * n;
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int k = 0; k < n; ++k) {
+    // pivot search over rows k..n-1 of column k
+    double best = -1.0;
+    int bi = k;
+    for (int r = k + threadIdx.x; r < n; r += blockDim.x) {
+      const double a = fabs(aug[(long long)r * w + k]);
+      if (a > best) { best = a; bi = r; }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, kWave);
+      const int oi = __shfl_xor(bi, off, kWave);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    if (lane == 0) { red_v[wid] = best; red_i[wid] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b = -1.0;
+      int i = k;
+      for (int q = 0; q < (int)(blockDim.x / kWave); ++q)
+        if (red_v[q] > b || (red_v[q] == b && red_i[q] < i)) { b = red_v[q]; i = red_i[q]; }
+      piv_row = i;
+    }
+    __syncthreads();
+    const int p = piv_row;
+    if (p != k) {
+      for (long long c = threadIdx.x; c < w; c += blockDim.x) {
+        const double t = aug[(long long)k * w + c];
+        aug[(long long)k * w + c] = aug[(long long)p * w + c];
+        aug[(long long)p * w + c] = t;
+      }
+    }
+    __syncthreads();
+    double piv = aug[(long long)k * w + k];
+    if (fabs(piv) < 1e-300) piv = 1e-300;  // singular pivot: keep the elimination finite
+    const double inv = 1.0 / piv;
+    __syncthreads();
+    for (long long c = k + threadIdx.x; c < w; c += blockDim.x) aug[(long long)k * w + c] *= inv;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) fcol[r] = (r == k) ? 0.0 : aug[(long long)r * w + k];
+    __syncthreads();
+    const long long span = w - k;
+    for (long long t = threadIdx.x; t < (long long)n * span; t += blockDim.x) {
+      const int r = (int)(t / span);
+      const long long c = k + t % span;
+      const double f = fcol[r];
+      if (f != 0.0) aug[(long long)r * w + c] -= f * aug[(long long)k * w + c];
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_solve(int n, const double* __restrict__ aug, const T* __restrict__ b,
+                                                     T* __restrict__ x) {
+  const long long w = 2LL * n;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int nwaves = (gridDim.x * blockDim.x) / kWave;
+  for (int r = wid; r < n; r += nwaves) {
+    double s = 0.0;
+    for (int c = 1 + lane; c < n; c += kWave) s += aug[(long long)r * w + n + c] * (double)b[c];
+    s = wave_sum(s);
+    if (lane == 0) x[r] = (T)s;
+  }
+}
+
+}  // namespace
+
+void launch_csr_to_dense_pinned(int n, const int* rp, const int* ci, const double* v, double* aug, hipStream_t s) {
+  IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
+  const long long tot = 2LL * n * n;
+  k_csr_to_dense_pinned<<<grid_for(tot, 256), 256, 0, s>>>(n, rp, ci, v, aug);
+  IMPES_LAUNCH_CHECK();
+  k_dense_scatter<<<grid_for(n, 256), 256, 0, s>>>(n, rp, ci, v, aug);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s) {
+  IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
+  k_gauss_jordan<<<1, 1024, 0, s>>>(n, aug);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_dense_solve_f64(int n, const double* aug, const double* b, double* x, hipStream_t s) {
+  k_dense_solve<double><<<div_up(n, 4), 256, 0, s>>>(n, aug, b, x);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_dense_solve_f32(int n, const double* aug, const float* b, float* x, hipStream_t s) {
+  k_dense_solve<float><<<div_up(n, 4), 256, 0, s>>>(n, aug, b, x);
+  IMPES_LAUNCH_CHECK();
+}
+
+}  // namespace impes

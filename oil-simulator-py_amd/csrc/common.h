@@ -1,0 +1,176 @@
+// Shared definitions for the IMPES HIP library (gfx950 / CDNA4 only).
+//
+// Conventions
+//  * every extern "C" entry point returns int (0 = ok, <0 = error) and records a
+//    message retrievable through impes_last_error() (thread-local);
+//  * all device work goes on the caller's hipStream_t;
+//  * indices are int32 (the largest operator handled, nnz(A0) = 6.94e6 at
+//    100x100x100, is far below 2^31), values are double or float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace impes {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// ---------------------------------------------------------------- errors
+struct Error : public std::runtime_error {
+  int code;
+  explicit Error(const std::string& m, int c = -1) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define IMPES_HIP_CHECK(expr)                                                     \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      throw ::impes::Error(std::string("HIP error '") + hipGetErrorString(_e) +  \
+                           "' at " + __FILE__ + ":" + std::to_string(__LINE__) + \
+                           " in " #expr, -2);                                    \
+    }                                                                             \
+  } while (0)
+
+#define IMPES_REQUIRE(cond, msg)                                   \
+  do {                                                             \
+    if (!(cond)) throw ::impes::Error(std::string(msg), -3);       \
+  } while (0)
+
+#define IMPES_LAUNCH_CHECK() IMPES_HIP_CHECK(hipGetLastError())
+
+// Wraps a C-ABI body: converts exceptions into an error code + message.
+template <class F>
+int guarded(const char* fn, F&& body) {
+  try {
+    body();
+    return 0;
+  } catch (const Error& e) {
+    set_last_error(std::string(fn) + ": " + e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(std::string(fn) + ": " + e.what());
+    return -1;
+  } catch (...) {
+    set_last_error(std::string(fn) + ": unknown exception");
+    return -1;
+  }
+}
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Grid size for a grid-stride loop over n items with `block` threads each,
+// capped so a launch is large enough to fill 256 CUs but no larger.
+inline int grid_for(long long n, int block, int cap = 8192) {
+  long long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// ---------------------------------------------------------------- device helpers
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Sum within aligned groups of G lanes (G a power of two <= 64).
+template <int G, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Block-wide sum; every thread gets the result. `lds` needs blockDim/64 slots.
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  T t = (threadIdx.x < nw) ? lds[threadIdx.x] : T(0);
+  if (wid == 0) t = wave_sum(t);
+  if (threadIdx.x == 0) lds[0] = t;
+  __syncthreads();
+  T r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ T block_max(T v, T* lds, T init) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  T t = (threadIdx.x < nw) ? lds[threadIdx.x] : init;
+  if (wid == 0) t = wave_max(t);
+  if (threadIdx.x == 0) lds[0] = t;
+  __syncthreads();
+  T r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------- device buffers
+// Minimal RAII owner of a device allocation (hipMalloc); used by handles that own
+// hierarchy / workspace memory for their lifetime.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    IMPES_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  // grow-only allocation (keeps the buffer when already large enough)
+  void reserve(size_t count) {
+    if (count <= n && p) return;
+    alloc(count);
+  }
+  T* get() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace impes

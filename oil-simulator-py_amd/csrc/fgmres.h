@@ -1,0 +1,49 @@
+// Restarted flexible GMRES in FP64 with classical Gram-Schmidt and one
+// reorthogonalisation pass (CGS2), right-preconditioned by the AMG V-cycle
+// (FP64 or FP32 hierarchy). The Krylov method runs on the row-scaled system
+// D^-1 A x = D^-1 b (D = diag A): its residual is a per-cell pressure
+// correction in Pa, so "relative residual" compares like with like even when
+// well rows exceed interior rows by 15 orders of magnitude.
+#pragma once
+
+#include "amg.h"
+
+namespace impes {
+
+struct Operator {
+  bool stencil = false;
+  Stencil7<double> st;
+  CsrF64 csr;
+  int group = 1;
+  const double* dinv = nullptr;  // 1/diag(A), length n
+  long long n = 0;
+};
+
+struct SolveStats {
+  int iterations = 0;      // total Arnoldi steps
+  int restarts = 0;
+  int converged = 0;
+  int precision = 64;      // hierarchy precision used by the preconditioner
+  double rel_residual = 0; // true ||D^-1(b-Ax)|| / ||D^-1 b|| at exit
+  double bnorm = 0;        // ||D^-1 b||
+  double r0_rel = 0;       // relative residual of the initial guess
+};
+
+class Fgmres {
+ public:
+  void solve(const Operator& A, Hierarchy* M, int precision, const double* b, double* x, bool use_x0, double rtol,
+             int restart, int maxit, SolveStats& st, hipStream_t s);
+
+ private:
+  void ensure(long long n, int m);
+  void apply(const Operator& A, const double* x, double* y, hipStream_t s);
+  void residual(const Operator& A, const double* x, const double* b, double* r, hipStream_t s);
+  long long n_ = 0;
+  int m_ = 0;
+  DevBuf<double> V_, Z_, w_, bs_, small_, work_;
+  double* host_ = nullptr;  // pinned scalars
+ public:
+  ~Fgmres();
+};
+
+}  // namespace impes

@@ -1,0 +1,88 @@
+// Host-side launchers of the HIP kernels (one translation unit per kernel family).
+// All launchers enqueue on the given stream and never synchronise.
+#pragma once
+
+#include "common.h"
+
+namespace impes {
+
+// ------------------------------------------------------------------ blas.hip
+constexpr int kRedBlocks = 512;  // fixed partial count of two-stage reductions
+constexpr int kMaxMultiDot = 128;
+
+void launch_fill_f64(double* x, double v, long long n, hipStream_t s);
+void launch_fill_f32(float* x, float v, long long n, hipStream_t s);
+void launch_fill_i32(int* x, int v, long long n, hipStream_t s);
+void launch_copy_f64(double* y, const double* x, long long n, hipStream_t s);
+void launch_copy_f32(float* y, const float* x, long long n, hipStream_t s);
+void launch_f64_to_f32(float* y, const double* x, long long n, hipStream_t s);
+// y = a .* x (elementwise), converting precision as needed
+void launch_scale_f64_to_f32(float* y, const double* a, const double* x, long long n, hipStream_t s);
+void launch_scale_f32_to_f64(double* y, const double* a, const float* x, long long n, hipStream_t s);
+void launch_scale_f64(double* y, const double* a, const double* x, long long n, hipStream_t s);
+// y += alpha * x  (alpha host scalar)
+void launch_axpy_f64(double* y, double alpha, const double* x, long long n, hipStream_t s);
+// out[0] = sum(x .* y); work >= kRedBlocks doubles
+void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s);
+// out[0] = max |x|
+void launch_amax_f64(const double* x, long long n, double* out, double* work, hipStream_t s);
+// h[i] = V_i . w  for i < k   (V_i = V + i*ldv); work >= kRedBlocks*k doubles
+void launch_multidot_f64(const double* V, long long ldv, int k, const double* w, long long n,
+                         double* h, double* work, hipStream_t s);
+// w -= sum_i V_i h[i]; if hacc != null also hacc[i] += h[i] (done by block 0)
+void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h,
+                          long long n, hipStream_t s);
+// v_out = w / nrm[0]   (nrm device scalar)
+void launch_scale_by_inv_f64(double* v_out, const double* w, const double* nrm, long long n, hipStream_t s);
+// x += sum_i Z_i y[i]
+void launch_lincomb_add_f64(double* x, const double* Z, long long ldz, int k, const double* y,
+                            long long n, hipStream_t s);
+
+// ------------------------------------------------------------------ spmv.hip
+struct CsrF64 { int n_rows = 0, n_cols = 0, nnz = 0; const int* rp = nullptr; const int* ci = nullptr; const double* v = nullptr; };
+struct CsrF32 { int n_rows = 0, n_cols = 0, nnz = 0; const int* rp = nullptr; const int* ci = nullptr; const float* v = nullptr; };
+
+// Structured 7-point symmetric operator on an (nx,ny,nz) grid, z fastest:
+// A[c,c] = d[c]; A[c,c+sx] = A[c+sx,c] = ox[c]; A[c,c+nz] = oy[c]; A[c,c+1] = oz[c].
+// ox/oy/oz hold 0 on the faces that leave the grid.
+template <class T>
+struct Stencil7 { int nx = 0, ny = 0, nz = 0; const T* d = nullptr; const T* ox = nullptr; const T* oy = nullptr; const T* oz = nullptr; };
+
+// pick lanes-per-row from the mean row length
+int csr_group_for(int n_rows, int nnz);
+
+void launch_spmv_f64(const CsrF64& A, const double* x, double* y, int group, hipStream_t s);
+void launch_spmv_f32(const CsrF32& A, const float* x, float* y, int group, hipStream_t s);
+// y += A x
+void launch_spmv_add_f64(const CsrF64& A, const double* x, double* y, int group, hipStream_t s);
+void launch_spmv_add_f32(const CsrF32& A, const float* x, float* y, int group, hipStream_t s);
+// r = b - A x
+void launch_residual_f64(const CsrF64& A, const double* x, const double* b, double* r, int group, hipStream_t s);
+void launch_residual_f32(const CsrF32& A, const float* x, const float* b, float* r, int group, hipStream_t s);
+// xo = xi + w * inv_relax .* (b - A xi)
+void launch_jacobi_f64(const CsrF64& A, const double* xi, const double* b, const double* inv_relax,
+                       double w, double* xo, int group, hipStream_t s);
+void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const float* inv_relax,
+                       float w, float* xo, int group, hipStream_t s);
+// xo = w * inv_relax .* b  (first sweep from a zero guess)
+void launch_jacobi0_f64(const double* b, const double* inv_relax, double w, double* xo, int n, hipStream_t s);
+void launch_jacobi0_f32(const float* b, const float* inv_relax, float w, float* xo, int n, hipStream_t s);
+
+void launch_st7_spmv_f64(const Stencil7<double>& S, const double* x, double* y, hipStream_t s);
+// r = b - A x
+void launch_st7_residual_f64(const Stencil7<double>& S, const double* x, const double* b, double* r, hipStream_t s);
+void launch_st7_residual_f32(const Stencil7<float>& S, const float* x, const float* b, float* r, hipStream_t s);
+void launch_st7_jacobi_f64(const Stencil7<double>& S, const double* xi, const double* b, const double* inv_relax,
+                           double w, double* xo, hipStream_t s);
+void launch_st7_jacobi_f32(const Stencil7<float>& S, const float* xi, const float* b, const float* inv_relax,
+                           float w, float* xo, hipStream_t s);
+// y = dinv .* (A x)   (row-scaled operator used by the outer Krylov method)
+void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv, const double* x, double* y, hipStream_t s);
+void launch_csr_rowscaled_spmv_f64(const CsrF64& A, const double* dinv, const double* x, double* y, int group, hipStream_t s);
+// r = dinv .* (b - A x)
+void launch_st7_rowscaled_residual_f64(const Stencil7<double>& S, const double* dinv, const double* x,
+                                       const double* b, double* r, hipStream_t s);
+void launch_csr_rowscaled_residual_f64(const CsrF64& A, const double* dinv, const double* x, const double* b,
+                                       double* r, int group, hipStream_t s);
+
+}  // namespace impes

@@ -1,0 +1,199 @@
+// Sparse operator kernels of the V-cycle and of the outer Krylov method.
+//
+// CSR: G lanes of a wavefront share one row (G = 1..64 chosen from the mean row
+// length), lanes walk the row with stride G so a group's loads are contiguous,
+// and the partial sums fold with cross-lane shuffles. Every variant is fused
+// with the vector update it feeds (residual, L1-Jacobi sweep, prolongation
+// add, row scaling) so each sweep streams the matrix once.
+//
+// Stencil7: the level-0 IMPES pressure operator is a symmetric 7-point stencil,
+// so it is stored index-free as one diagonal and three face-coefficient arrays
+// (4 values per cell instead of CSR's ~7 values + 7 indices + row pointers).
+#include "kernels.h"
+
+namespace impes {
+
+namespace {
+constexpr int kBlock = 256;
+
+enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5 };
+
+template <int G, class T, class TA, int MODE>
+__global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const T* __restrict__ v, const T* __restrict__ x,
+                                                 const T* __restrict__ b, const TA* __restrict__ aux, T w,
+                                                 T* __restrict__ y) {
+  constexpr int RPB = kBlock / G;  // rows per block
+  const int g = threadIdx.x % G;
+  for (long long base = (long long)blockIdx.x * RPB; base < n; base += (long long)gridDim.x * RPB) {
+    const long long row = base + threadIdx.x / G;
+    const bool active = row < n;
+    T sum = T(0);
+    if (active) {
+      const int e1 = rp[row + 1];
+      for (int e = rp[row] + g; e < e1; e += G) sum += v[e] * x[ci[e]];
+    }
+    if (G > 1) sum = group_sum<G>(sum);
+    if (active && g == 0) {
+      if (MODE == kY) y[row] = sum;
+      else if (MODE == kYAdd) y[row] += sum;
+      else if (MODE == kRes) y[row] = b[row] - sum;
+      else if (MODE == kJac) y[row] = x[row] + w * (T)aux[row] * (b[row] - sum);
+      else if (MODE == kRowScaled) y[row] = (T)aux[row] * sum;
+      else if (MODE == kRowScaledRes) y[row] = (T)aux[row] * (b[row] - sum);
+    }
+  }
+}
+
+template <class T, class TA, int MODE>
+void csr_dispatch(int n, const int* rp, const int* ci, const T* v, const T* x, const T* b, const TA* aux, T w, T* y,
+                  int G, hipStream_t s) {
+  if (n <= 0) return;
+  const int rpb = kBlock / G;
+  const int grid = grid_for(n, rpb, 1 << 18);
+  switch (G) {
+    case 1: k_csr<1, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 2: k_csr<2, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 4: k_csr<4, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 8: k_csr<8, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 16: k_csr<16, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 32: k_csr<32, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    default: k_csr<64, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+  }
+  IMPES_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- stencil 7
+template <class T>
+struct St7Row {
+  // sum_j A[c,j] x[j] in the column order of the CSR form (c-sx, c-nz, c-1, c, c+1, c+nz, c+sx)
+  __device__ __forceinline__ static T apply(const Stencil7<T>& S, const T* __restrict__ x, long long c, int i, int j,
+                                            int k) {
+    const long long sy = S.nz;
+    const long long sx = (long long)S.ny * S.nz;
+    T s = T(0);
+    if (i > 0) s += S.ox[c - sx] * x[c - sx];
+    if (j > 0) s += S.oy[c - sy] * x[c - sy];
+    if (k > 0) s += S.oz[c - 1] * x[c - 1];
+    s += S.d[c] * x[c];
+    if (k < S.nz - 1) s += S.oz[c] * x[c + 1];
+    if (j < S.ny - 1) s += S.oy[c] * x[c + sy];
+    if (i < S.nx - 1) s += S.ox[c] * x[c + sx];
+    return s;
+  }
+};
+
+template <class T, class TA, int MODE>
+__global__ __launch_bounds__(kBlock) void k_st7(Stencil7<T> S, const T* __restrict__ x, const T* __restrict__ b,
+                                                const TA* __restrict__ aux, T w, T* __restrict__ y) {
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(c % S.nz);
+    const long long t = c / S.nz;
+    const int j = (int)(t % S.ny);
+    const int i = (int)(t / S.ny);
+    const T sum = St7Row<T>::apply(S, x, c, i, j, k);
+    if (MODE == kY) y[c] = sum;
+    else if (MODE == kRes) y[c] = b[c] - sum;
+    else if (MODE == kJac) y[c] = x[c] + w * (T)aux[c] * (b[c] - sum);
+    else if (MODE == kRowScaled) y[c] = (T)aux[c] * sum;
+    else if (MODE == kRowScaledRes) y[c] = (T)aux[c] * (b[c] - sum);
+  }
+}
+
+template <class T>
+__global__ void k_jacobi0(const T* __restrict__ b, const T* __restrict__ inv_relax, T w, T* __restrict__ xo, int n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    xo[i] = w * inv_relax[i] * b[i];
+}
+
+template <class T, class TA, int MODE>
+void st7_launch(const Stencil7<T>& S, const T* x, const T* b, const TA* aux, T w, T* y, hipStream_t s) {
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  if (n <= 0) return;
+  k_st7<T, TA, MODE><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(S, x, b, aux, w, y);
+  IMPES_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+int csr_group_for(int n_rows, int nnz) {
+  if (n_rows <= 0) return 1;
+  const double mean = (double)nnz / (double)n_rows;
+  int g = 1;
+  while (g < 64 && g < mean * 0.75) g <<= 1;
+  return g;
+}
+
+void launch_spmv_f64(const CsrF64& A, const double* x, double* y, int G, hipStream_t s) {
+  csr_dispatch<double, double, kY>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, (const double*)nullptr, 0.0, y, G, s);
+}
+void launch_spmv_f32(const CsrF32& A, const float* x, float* y, int G, hipStream_t s) {
+  csr_dispatch<float, float, kY>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, (const float*)nullptr, 0.f, y, G, s);
+}
+void launch_spmv_add_f64(const CsrF64& A, const double* x, double* y, int G, hipStream_t s) {
+  csr_dispatch<double, double, kYAdd>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, (const double*)nullptr, 0.0, y, G, s);
+}
+void launch_spmv_add_f32(const CsrF32& A, const float* x, float* y, int G, hipStream_t s) {
+  csr_dispatch<float, float, kYAdd>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, (const float*)nullptr, 0.f, y, G, s);
+}
+void launch_residual_f64(const CsrF64& A, const double* x, const double* b, double* r, int G, hipStream_t s) {
+  csr_dispatch<double, double, kRes>(A.n_rows, A.rp, A.ci, A.v, x, b, (const double*)nullptr, 0.0, r, G, s);
+}
+void launch_residual_f32(const CsrF32& A, const float* x, const float* b, float* r, int G, hipStream_t s) {
+  csr_dispatch<float, float, kRes>(A.n_rows, A.rp, A.ci, A.v, x, b, (const float*)nullptr, 0.f, r, G, s);
+}
+void launch_jacobi_f64(const CsrF64& A, const double* xi, const double* b, const double* inv_relax, double w,
+                       double* xo, int G, hipStream_t s) {
+  csr_dispatch<double, double, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
+}
+void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const float* inv_relax, float w, float* xo,
+                       int G, hipStream_t s) {
+  csr_dispatch<float, float, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
+}
+void launch_jacobi0_f64(const double* b, const double* inv_relax, double w, double* xo, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_jacobi0<double><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(b, inv_relax, w, xo, n);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_jacobi0_f32(const float* b, const float* inv_relax, float w, float* xo, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_jacobi0<float><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(b, inv_relax, w, xo, n);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_csr_rowscaled_spmv_f64(const CsrF64& A, const double* dinv, const double* x, double* y, int G,
+                                   hipStream_t s) {
+  csr_dispatch<double, double, kRowScaled>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, dinv, 0.0, y, G, s);
+}
+void launch_csr_rowscaled_residual_f64(const CsrF64& A, const double* dinv, const double* x, const double* b,
+                                       double* r, int G, hipStream_t s) {
+  csr_dispatch<double, double, kRowScaledRes>(A.n_rows, A.rp, A.ci, A.v, x, b, dinv, 0.0, r, G, s);
+}
+
+void launch_st7_spmv_f64(const Stencil7<double>& S, const double* x, double* y, hipStream_t s) {
+  st7_launch<double, double, kY>(S, x, nullptr, (const double*)nullptr, 0.0, y, s);
+}
+void launch_st7_residual_f64(const Stencil7<double>& S, const double* x, const double* b, double* r, hipStream_t s) {
+  st7_launch<double, double, kRes>(S, x, b, (const double*)nullptr, 0.0, r, s);
+}
+void launch_st7_residual_f32(const Stencil7<float>& S, const float* x, const float* b, float* r, hipStream_t s) {
+  st7_launch<float, float, kRes>(S, x, b, (const float*)nullptr, 0.f, r, s);
+}
+void launch_st7_jacobi_f64(const Stencil7<double>& S, const double* xi, const double* b, const double* inv_relax,
+                           double w, double* xo, hipStream_t s) {
+  st7_launch<double, double, kJac>(S, xi, b, inv_relax, w, xo, s);
+}
+void launch_st7_jacobi_f32(const Stencil7<float>& S, const float* xi, const float* b, const float* inv_relax, float w,
+                           float* xo, hipStream_t s) {
+  st7_launch<float, float, kJac>(S, xi, b, inv_relax, w, xo, s);
+}
+void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv, const double* x, double* y,
+                                   hipStream_t s) {
+  st7_launch<double, double, kRowScaled>(S, x, nullptr, dinv, 0.0, y, s);
+}
+void launch_st7_rowscaled_residual_f64(const Stencil7<double>& S, const double* dinv, const double* x,
+                                       const double* b, double* r, hipStream_t s) {
+  st7_launch<double, double, kRowScaledRes>(S, x, b, dinv, 0.0, r, s);
+}
+
+}  // namespace impes

@@ -1,0 +1,142 @@
+"""ctypes binding of libimpes_hip.so (declarations mirror include/impes_hip.h).
+
+The library is loaded after ``import torch`` so that it binds to the HIP runtime
+torch already loaded (both carry the SONAME ``libamdhip64.so.7``): one runtime,
+one set of streams, device pointers shared freely.  There is no fallback: if the
+shared library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import torch  # noqa: F401  (loads the HIP runtime first)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libimpes_hip.so")
+
+_lock = threading.Lock()
+_lib = None
+
+
+class ImpesError(RuntimeError):
+    """Error reported by the HIP library (negative status code)."""
+
+
+class AmgParams(C.Structure):
+    _fields_ = [
+        ("theta", C.c_double),
+        ("max_levels", C.c_int),
+        ("coarsest_size", C.c_int),
+        ("pre_smooth", C.c_int),
+        ("post_smooth", C.c_int),
+        ("omega", C.c_double),
+        ("equil_threshold", C.c_double),
+        ("dense_max", C.c_int),
+        ("coarse_sweeps", C.c_int),
+        ("anchor", C.c_int),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("restarts", C.c_int),
+        ("converged", C.c_int),
+        ("precision", C.c_int),
+        ("rel_residual", C.c_double),
+        ("bnorm", C.c_double),
+        ("r0_rel", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_longlong
+_D = C.c_double
+_F = C.c_float
+
+# name -> argtypes ; every function returns int unless listed in _RESTYPE
+SIGNATURES = {
+    "impes_abi_version": [],
+    "impes_last_error": [],
+    "impes_spmv_csr_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P],
+    "impes_spmv_csr_f32": [_I, _I, _I, _P, _P, _P, _P, _P, _P],
+    "impes_spmv_st7_f64": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+    "impes_l1jacobi_csr_f64": [_I, _I, _P, _P, _P, _P, _D, _P, _P, _P, _P],
+    "impes_l1jacobi_csr_f32": [_I, _I, _P, _P, _P, _P, _F, _P, _P, _P, _P],
+    "impes_dot_f64": [_L, _P, _P, _P, _P],
+    "impes_amg_default_params": [C.POINTER(AmgParams)],
+    "impes_amg_create": [C.POINTER(AmgParams), C.POINTER(_P)],
+    "impes_amg_destroy": [_P],
+    "impes_amg_setup_csr": [_P, _I, _I, _P, _P, _P, _P],
+    "impes_amg_setup_st7": [_P, _I, _I, _I, _P, _P, _P, _P, _P],
+    "impes_amg_update_csr": [_P, _P, _P],
+    "impes_amg_update_st7": [_P, _P, _P, _P, _P, _P],
+    "impes_amg_num_levels": [_P, C.POINTER(_I)],
+    "impes_amg_level_info": [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
+    "impes_amg_equilibrated": [_P, C.POINTER(_I)],
+    "impes_amg_setup_seconds": [_P, C.POINTER(_D)],
+    "impes_amg_export_level": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "impes_amg_export_scaling": [_P, _P, _P],
+    "impes_amg_vcycle": [_P, _I, _P, _P, _I, _P],
+    "impes_amg_precondition": [_P, _I, _P, _P, _P],
+    "impes_solver_create": [C.POINTER(_P)],
+    "impes_solver_destroy": [_P],
+    "impes_fgmres_csr": [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I, C.POINTER(SolveStats), _P],
+    "impes_fgmres_st7": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I,
+                         C.POINTER(SolveStats), _P],
+}
+_RESTYPE = {"impes_last_error": C.c_char_p, "impes_amg_default_params": None}
+
+
+def load():
+    """Load and declare the library (idempotent). Raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImpesError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `python oil-simulator-py_amd/csrc/build.py`")
+        lib = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int)
+        _lib = lib
+        return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(SIGNATURES)
+
+
+def call(name: str, *args):
+    """Call an int-returning entry point; raise ImpesError on a negative status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.impes_last_error()
+        raise ImpesError(f"{name} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+def ptr(t) -> C.c_void_p | None:
+    """Device pointer of a CUDA tensor (None for None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ImpesError("IMPES HIP kernels need tensors on a ROCm (cuda) device")
+    if not t.is_contiguous():
+        raise ImpesError("IMPES HIP kernels need contiguous tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None) -> C.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)

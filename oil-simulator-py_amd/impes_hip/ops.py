@@ -1,0 +1,224 @@
+"""Tensor-level wrappers of the C ABI (torch tensors on a ROCm device in, out).
+
+Index tensors are int32, value tensors float64 or float32, all contiguous and on
+the same device. Launches go on the current torch stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from ._lib import AmgParams, ImpesError, SolveStats, call, load, ptr, stream_ptr
+
+__all__ = [
+    "spmv_csr", "spmv_st7", "l1jacobi_csr", "dot", "AmgHierarchy", "FgmresSolver", "amg_default_params",
+]
+
+
+def _i32(t):
+    if t.dtype != torch.int32:
+        raise ImpesError(f"expected int32 indices, got {t.dtype}")
+    return t
+
+
+def spmv_csr(rowptr, col, val, x, n_cols=None, out=None):
+    n = rowptr.numel() - 1
+    n_cols = x.numel() if n_cols is None else n_cols
+    y = out if out is not None else torch.empty(n, dtype=val.dtype, device=val.device)
+    fn = "impes_spmv_csr_f64" if val.dtype == torch.float64 else "impes_spmv_csr_f32"
+    if x.dtype != val.dtype:
+        raise ImpesError("x and val dtypes differ")
+    call(fn, n, n_cols, val.numel(), ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(x), ptr(y), stream_ptr())
+    return y
+
+
+def spmv_st7(dims, d, ox, oy, oz, x, out=None):
+    nx, ny, nz = (int(v) for v in dims)
+    y = out if out is not None else torch.empty_like(x)
+    call("impes_spmv_st7_f64", nx, ny, nz, ptr(d), ptr(ox), ptr(oy), ptr(oz), ptr(x), ptr(y), stream_ptr())
+    return y
+
+
+def l1jacobi_csr(rowptr, col, val, inv_relax, omega, x_in, b, out=None):
+    n = rowptr.numel() - 1
+    y = out if out is not None else torch.empty_like(x_in)
+    if val.dtype == torch.float64:
+        call("impes_l1jacobi_csr_f64", n, val.numel(), ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val),
+             ptr(inv_relax), float(omega), ptr(x_in), ptr(b), ptr(y), stream_ptr())
+    else:
+        call("impes_l1jacobi_csr_f32", n, val.numel(), ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val),
+             ptr(inv_relax), C.c_float(omega), ptr(x_in), ptr(b), ptr(y), stream_ptr())
+    return y
+
+
+def dot(x, y):
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    call("impes_dot_f64", x.numel(), ptr(x), ptr(y), ptr(out), stream_ptr())
+    return out
+
+
+def amg_default_params() -> AmgParams:
+    p = AmgParams()
+    load().impes_amg_default_params(C.byref(p))
+    return p
+
+
+class AmgHierarchy:
+    """Owner of an ``impes_amg`` handle (device-resident RS-AMG hierarchy)."""
+
+    def __init__(self, theta=0.25, max_levels=10, coarsest_size=200, pre_smooth=3, post_smooth=2, omega=0.7,
+                 equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, anchor=None):
+        p = amg_default_params()
+        p.theta = float(theta)
+        p.max_levels = int(max_levels)
+        p.coarsest_size = int(coarsest_size)
+        p.pre_smooth = int(pre_smooth)
+        p.post_smooth = int(post_smooth)
+        p.omega = float(omega)
+        p.equil_threshold = float(equil_threshold)
+        p.dense_max = int(dense_max)
+        p.coarse_sweeps = int(coarse_sweeps)
+        p.anchor = -1 if anchor is None else int(anchor)
+        self.params = p
+        h = C.c_void_p()
+        call("impes_amg_create", C.byref(p), C.byref(h))
+        self._h = h
+        self.n = 0
+        self.device = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            try:
+                load().impes_amg_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def setup_csr(self, rowptr, col, val):
+        n = rowptr.numel() - 1
+        call("impes_amg_setup_csr", self._h, n, val.numel(), ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val),
+             stream_ptr())
+        self.n = n
+        self.device = val.device
+
+    def setup_st7(self, dims, d, ox, oy, oz):
+        nx, ny, nz = (int(v) for v in dims)
+        call("impes_amg_setup_st7", self._h, nx, ny, nz, ptr(d), ptr(ox), ptr(oy), ptr(oz), stream_ptr())
+        self.n = nx * ny * nz
+        self.device = d.device
+
+    def update_csr(self, val):
+        call("impes_amg_update_csr", self._h, ptr(val), stream_ptr())
+
+    def update_st7(self, d, ox, oy, oz):
+        call("impes_amg_update_st7", self._h, ptr(d), ptr(ox), ptr(oy), ptr(oz), stream_ptr())
+
+    @property
+    def num_levels(self) -> int:
+        v = C.c_int()
+        call("impes_amg_num_levels", self._h, C.byref(v))
+        return v.value
+
+    def level_info(self, level: int) -> dict:
+        n, nnz, pnnz, co, de = (C.c_int() for _ in range(5))
+        call("impes_amg_level_info", self._h, int(level), C.byref(n), C.byref(nnz), C.byref(pnnz), C.byref(co),
+             C.byref(de))
+        return {"n": n.value, "nnz": nnz.value, "p_nnz": pnnz.value, "is_coarsest": bool(co.value),
+                "dense": bool(de.value)}
+
+    @property
+    def equilibrated(self) -> bool:
+        v = C.c_int()
+        call("impes_amg_equilibrated", self._h, C.byref(v))
+        return bool(v.value)
+
+    @property
+    def setup_seconds(self) -> float:
+        v = C.c_double()
+        call("impes_amg_setup_seconds", self._h, C.byref(v))
+        return v.value
+
+    def export_level(self, level: int) -> dict:
+        info = self.level_info(level)
+        dev = self.device
+        n, nnz, pnnz = info["n"], info["nnz"], info["p_nnz"]
+        out = {
+            "rowptr": torch.empty(n + 1, dtype=torch.int32, device=dev),
+            "col": torch.empty(nnz, dtype=torch.int32, device=dev),
+            "val": torch.empty(nnz, dtype=torch.float64, device=dev),
+            "inv_relax": torch.empty(n, dtype=torch.float64, device=dev),
+        }
+        if not info["is_coarsest"]:
+            out["p_rowptr"] = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            out["p_col"] = torch.empty(pnnz, dtype=torch.int32, device=dev)
+            out["p_val"] = torch.empty(pnnz, dtype=torch.float64, device=dev)
+            out["cf"] = torch.empty(n, dtype=torch.int8, device=dev)
+        call("impes_amg_export_level", self._h, int(level), ptr(out["rowptr"]), ptr(out["col"]), ptr(out["val"]),
+             ptr(out.get("p_rowptr")), ptr(out.get("p_col")), ptr(out.get("p_val")), ptr(out.get("cf")),
+             ptr(out["inv_relax"]), stream_ptr())
+        out.update(info)
+        return out
+
+    def dhalf_inv(self):
+        t = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        call("impes_amg_export_scaling", self._h, ptr(t), stream_ptr())
+        return t
+
+    def vcycle(self, b, x=None, precision=64):
+        """x <- V(b) in the equilibrated space (x=None: start from zero)."""
+        zero = x is None
+        if zero:
+            x = torch.empty_like(b)
+        call("impes_amg_vcycle", self._h, int(precision), ptr(b), ptr(x), 1 if zero else 0, stream_ptr())
+        return x
+
+    def precondition(self, v, precision=64, out=None):
+        z = out if out is not None else torch.empty_like(v)
+        call("impes_amg_precondition", self._h, int(precision), ptr(v), ptr(z), stream_ptr())
+        return z
+
+
+class FgmresSolver:
+    """Owner of an ``impes_solver`` handle (FGMRES workspace)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("impes_solver_create", C.byref(h))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            try:
+                load().impes_solver_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve_csr(self, amg, rowptr, col, val, dinv, b, x, *, precision=64, use_x0=False, rtol=1e-8,
+                  restart=30, maxit=300) -> dict:
+        st = SolveStats()
+        n = rowptr.numel() - 1
+        call("impes_fgmres_csr", self._h, amg.handle if amg is not None else None, int(precision), n, val.numel(),
+             ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(dinv), ptr(b), ptr(x), 1 if use_x0 else 0,
+             float(rtol), int(restart), int(maxit), C.byref(st), stream_ptr())
+        return st.as_dict()
+
+    def solve_st7(self, amg, dims, d, ox, oy, oz, dinv, b, x, *, precision=64, use_x0=False, rtol=1e-8,
+                  restart=30, maxit=300) -> dict:
+        st = SolveStats()
+        nx, ny, nz = (int(v) for v in dims)
+        call("impes_fgmres_st7", self._h, amg.handle if amg is not None else None, int(precision), nx, ny, nz,
+             ptr(d), ptr(ox), ptr(oy), ptr(oz), ptr(dinv), ptr(b), ptr(x), 1 if use_x0 else 0, float(rtol),
+             int(restart), int(maxit), C.byref(st), stream_ptr())
+        return st.as_dict()

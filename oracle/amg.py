@@ -1,0 +1,404 @@
+"""ORACLE (test infrastructure only): FP64 numpy/scipy restatement of the
+reference's Ruge-Stueben AMG and of the FGMRES method that replaces its
+stationary V-cycle iteration.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import this module, and only as the checker.
+
+Reference: src/solver/classical_amg.py (file:line cited per function).
+The reference's unseeded ``torch.rand`` C/F tie-break (classical_amg.py:125) is
+restated as the deterministic (degree, hash(index), index) order the GPU uses,
+so C/F splittings and interpolation patterns can be compared exactly.
+The reference's energy-minimising blend of interpolation weights
+(classical_amg.py:389-418) indexes the fine-level basis with coarse column
+numbers; it is not restated (documented deviation).
+
+Parity pinning of each function (checked in tests/test_oracle_amg.py):
+  strength            brute-force per-row loop on small matrices
+  cf_split            brute-force definition (strict local maxima + lift)
+  interpolation       P*1 = 1 on F rows, C rows identity, P >= 0, brute-force
+                      per-row restatement on small matrices
+  galerkin            dense R*A*P
+  vcycle / setup      reference KAT: 12x12 anchored Laplacian, 3 cycles of
+                      apply() reach ||Az-v||/||v|| < 5e-2 (test_amg_pressure_solver.py:89-106)
+  fgmres              scipy.sparse.linalg.spsolve on the same system
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+C_PT, F_PT = 0, 1
+
+
+def hash32(x: np.ndarray) -> np.ndarray:
+    """lowbias32 integer hash (same constants as the HIP kernel)."""
+    x = np.asarray(x, dtype=np.uint64) & np.uint64(0xFFFFFFFF)
+    m = np.uint64(0xFFFFFFFF)
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x7FEB352D)) & m
+    x ^= x >> np.uint64(15)
+    x = (x * np.uint64(0x846CA68B)) & m
+    x ^= x >> np.uint64(16)
+    return x.astype(np.uint32)
+
+
+def _rows(A: sp.csr_matrix) -> np.ndarray:
+    return np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+
+
+def _diag_first(A: sp.csr_matrix) -> np.ndarray:
+    """a_ii (first diagonal entry of each row, 0 if absent)."""
+    rows = _rows(A)
+    m = A.indices == rows
+    d = np.zeros(A.shape[0])
+    idx = np.nonzero(m)[0][::-1]  # reversed so the first occurrence is written last
+    d[rows[idx]] = A.data[idx]
+    return d
+
+
+def strength(A: sp.csr_matrix, theta: float) -> np.ndarray:
+    """classical_amg.py:34-82 — strong(i,j) iff a_ij<0, j!=i and
+    -a_ij/|a_ii| >= theta * max_k(-a_ik/|a_ii|)."""
+    rows = _rows(A)
+    d = np.abs(_diag_first(A))
+    sv = A.data / (d[rows] + 1e-30)
+    cand = (A.indices != rows) & (sv < 0)
+    neg_abs = np.where(cand, -sv, 0.0)
+    mx = np.zeros(A.shape[0])
+    np.maximum.at(mx, rows, neg_abs)
+    return cand & (neg_abs >= theta * mx[rows])
+
+
+def transpose_pos(A: sp.csr_matrix) -> np.ndarray:
+    n = A.shape[0]
+    rows = _rows(A).astype(np.int64)
+    keys = rows * n + A.indices
+    tkeys = A.indices.astype(np.int64) * n + rows
+    pos = np.searchsorted(keys, tkeys)
+    pos = np.minimum(pos, len(keys) - 1)
+    ok = keys[pos] == tkeys
+    return np.where(ok, pos, -1)
+
+
+def cf_split(A: sp.csr_matrix, strong: np.ndarray):
+    """classical_amg.py:85-146 with the deterministic tie-break. Returns
+    (cf markers before orphan promotion, number of C points)."""
+    n = A.shape[0]
+    rows = _rows(A)
+    tp = transpose_pos(A)
+    if np.any(tp < 0):
+        raise ValueError("matrix pattern is not structurally symmetric")
+    ing = (strong | strong[tp]) & (A.indices != rows)
+    deg = np.bincount(rows[ing], minlength=n)
+    h = hash32(np.arange(n))
+    order = np.lexsort((np.arange(n), h, deg))  # ascending by (deg, hash, idx)
+    rank = np.empty(n, dtype=np.int64)
+    rank[order] = np.arange(n)
+    nm = np.full(n, -1, dtype=np.int64)
+    np.maximum.at(nm, rows[ing], rank[A.indices[ing]])
+    is_max = rank > nm
+    has_c = np.zeros(n, dtype=bool)
+    np.logical_or.at(has_c, rows[ing], is_max[A.indices[ing]])
+    cf = np.where(is_max, C_PT, np.where(has_c, F_PT, C_PT)).astype(np.int8)
+    return cf, int(np.sum(cf == C_PT))
+
+
+def promote_orphans(A: sp.csr_matrix, strong: np.ndarray, cf: np.ndarray) -> np.ndarray:
+    """classical_amg.py:177-189 — F points without a strong C neighbour become C."""
+    rows = _rows(A)
+    m = strong & (A.indices != rows) & (cf[A.indices] == C_PT)
+    has = np.bincount(rows[m], minlength=A.shape[0]) > 0
+    out = cf.copy()
+    out[(cf == F_PT) & ~has] = C_PT
+    return out
+
+
+def interpolation(A: sp.csr_matrix, cf: np.ndarray, strong: np.ndarray) -> sp.csr_matrix:
+    """classical_amg.py:191-346 — RS direct weights with the weak-sum correction,
+    distance-two F-F contributions, clamping, F-row normalisation and the
+    zero-row fallback; explicit zeros are dropped."""
+    n = A.shape[0]
+    rows = _rows(A)
+    cols = A.indices
+    v = A.data
+    isC = cf == C_PT
+    isF = ~isC
+    f2c = np.cumsum(isC) - 1
+    nc = int(isC.sum())
+    off = cols != rows
+    m_sC = strong & off & isC[cols] & isF[rows]
+    m_wC = ~strong & off & isC[cols] & isF[rows]
+    m_wF = ~strong & off & isF[cols] & isF[rows]
+    m_sF = strong & off & isF[cols] & isF[rows]
+    sum_strong = np.bincount(rows[m_sC], weights=-v[m_sC], minlength=n)
+    cnt_c = np.bincount(rows[m_sC], minlength=n)
+    valid = (cnt_c > 0) & (sum_strong > 1e-14)
+    diag = np.zeros(n)
+    dm = cols == rows
+    np.add.at(diag, rows[dm], v[dm])
+    diag = np.where(np.abs(diag) > 1e-14, diag, 1.0)
+    inval = m_sF & ~valid[cols]
+    weak_c = np.bincount(rows[m_wC], weights=-v[m_wC], minlength=n)
+    f_weak = np.bincount(rows[m_wF], weights=-v[m_wF], minlength=n) + np.bincount(
+        rows[inval], weights=-v[inval], minlength=n)
+    corr = np.zeros(n)
+    mc = (sum_strong > 1e-14) & isF
+    corr[mc] = (weak_c[mc] + f_weak[mc]) / sum_strong[mc]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d_rows = rows[m_sC]
+        d_vals = np.nan_to_num(-v[m_sC] / diag[d_rows], nan=0.0, posinf=0.0, neginf=0.0)
+        d_vals = np.maximum(d_vals * (1.0 + corr[d_rows]), 0.0)
+    d_cols = f2c[cols[m_sC]]
+    # distance-two: strong C entries of each F row j, in row order
+    sc_idx = np.nonzero(m_sC)[0]
+    sc_ptr = np.concatenate([[0], np.cumsum(cnt_c)])
+    ff = np.nonzero(m_sF & valid[cols])[0]
+    if ff.size:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            factor = np.nan_to_num(-v[ff] / diag[rows[ff]], nan=0.0, posinf=0.0, neginf=0.0)
+        j = cols[ff]
+        cnt = cnt_c[j]
+        rep_rows = np.repeat(rows[ff], cnt)
+        rep_fac = np.repeat(factor, cnt)
+        rep_j = np.repeat(j, cnt)
+        base = np.repeat(sc_ptr[j], cnt)
+        intra = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        e2 = sc_idx[base + intra]
+        jw = -v[e2] / sum_strong[rep_j]
+        f_vals = np.maximum(rep_fac * jw, 0.0)
+        f_cols = f2c[cols[e2]]
+        f_rows = rep_rows
+    else:
+        f_vals = np.zeros(0)
+        f_cols = np.zeros(0, dtype=np.int64)
+        f_rows = np.zeros(0, dtype=np.int64)
+    c_rows = np.nonzero(isC)[0]
+    R = np.concatenate([c_rows, d_rows, f_rows])
+    Cc = np.concatenate([f2c[c_rows], d_cols, f_cols])
+    V = np.concatenate([np.ones(c_rows.size), d_vals, f_vals])
+    P = sp.coo_matrix((V, (R, Cc)), shape=(n, max(nc, 1))).tocsr()
+    P.sum_duplicates()
+    P.sort_indices()
+    rs = np.asarray(P.sum(axis=1)).ravel()
+    scale = np.ones(n)
+    good = isF & (rs > 1e-14)
+    scale[good] = 1.0 / rs[good]
+    zero_rows = np.nonzero(isF & ~(rs > 1e-14))[0]
+    if zero_rows.size:
+        P = P.tolil()
+        for i in zero_rows:
+            nb = A.indices[A.indptr[i]:A.indptr[i + 1]]
+            cn = nb[isC[nb]]
+            if cn.size == 0:
+                continue
+            t = f2c[cn[0]]
+            P[i, t] = P[i, t] + 1.0
+            s = P[i].sum()
+            scale[i] = 1.0 / s if s > 1e-14 else 1.0
+        P = P.tocsr()
+        P.sort_indices()
+    P = sp.diags(scale) @ P
+    P.data = np.maximum(P.data, 0.0)
+    P.eliminate_zeros()
+    P.sort_indices()
+    return P.tocsr()
+
+
+def inv_relax(A: sp.csr_matrix) -> np.ndarray:
+    """classical_amg.py:885-889: 1/max(|a_ii|, 0.3*sum_j|a_ij|), capped at 1e2."""
+    d = np.abs(_diag_first(A))
+    ra = np.add.reduceat(np.abs(A.data), A.indptr[:-1]) if A.nnz else np.zeros(A.shape[0])
+    ra[np.diff(A.indptr) == 0] = 0.0
+    den = np.maximum(np.maximum(d, 0.3 * ra), 1e-30)
+    return np.minimum(1.0 / den, 1e2)
+
+
+class Hierarchy:
+    """classical_amg.py:699-938 (setup) and :1037-1180 (cycles)."""
+
+    def __init__(self, A0: sp.csr_matrix, theta=0.25, max_levels=10, coarsest_size=200, pre=3, post=2,
+                 omega=0.7, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50):
+        self.theta, self.max_levels, self.coarsest_size = theta, max_levels, coarsest_size
+        self.pre, self.post, self.omega = pre, post, omega
+        self.equil_threshold, self.dense_max, self.coarse_sweeps = equil_threshold, dense_max, coarse_sweeps
+        A0 = sp.csr_matrix(A0, dtype=np.float64)
+        A0.sort_indices()
+        self._setup(A0)
+
+    def _equilibrate(self, A0):
+        d = np.maximum(np.abs(_diag_first(A0)), 1e-30)
+        med = np.sort(d)[(d.size - 1) // 2]  # torch.median: lower median
+        self.equilibrated = bool(med < self.equil_threshold)
+        self.diag0 = _diag_first(A0)
+        if self.equilibrated:
+            self.dh = 1.0 / np.sqrt(d)
+            Dh = sp.diags(self.dh)
+            A = (Dh @ A0 @ Dh).tocsr()
+            A.sort_indices()
+        else:
+            self.dh = np.ones(A0.shape[0])
+            A = A0.copy()
+        return A
+
+    def _setup(self, A0):
+        A = self._equilibrate(A0)
+        self.levels = []
+        for _lvl in range(self.max_levels):
+            n = A.shape[0]
+            if n <= self.coarsest_size:
+                break
+            s = strength(A, self.theta)
+            cf0, ncoarse = cf_split(A, s)
+            if ncoarse >= 0.9 * n:
+                break
+            cf = promote_orphans(A, s, cf0)
+            P = interpolation(A, cf, s)
+            R = P.T.tocsr()
+            R.sort_indices()
+            Ac = (R @ (A @ P)).tocsr()
+            Ac.sort_indices()
+            self.levels.append({"A": A, "P": P, "R": R, "cf": cf, "strong": s, "ir": inv_relax(A),
+                                "coarsest": False})
+            A = Ac
+        self.levels.append({"A": A, "P": None, "R": None, "cf": None, "ir": inv_relax(A), "coarsest": True})
+        self._coarse_factor()
+
+    def _coarse_factor(self):
+        L = self.levels[-1]
+        n = L["A"].shape[0]
+        L["dense"] = n <= self.dense_max
+        if L["dense"]:
+            Ad = L["A"].toarray()
+            Ad[0, :] = 0.0
+            Ad[:, 0] = 0.0
+            Ad[0, 0] = 1.0
+            L["Ainv"] = np.linalg.inv(Ad)
+
+    def update(self, A0: sp.csr_matrix):
+        """classical_amg.py:1308-1392: new values, frozen P."""
+        A0 = sp.csr_matrix(A0, dtype=np.float64)
+        A0.sort_indices()
+        A = self._equilibrate(A0)
+        for L in self.levels:
+            L["A"] = A
+            L["ir"] = inv_relax(A)
+            if L["coarsest"]:
+                break
+            A = (L["R"] @ (A @ L["P"])).tocsr()
+            A.sort_indices()
+        self._coarse_factor()
+
+    # ------------------------------------------------------------- cycles
+    def _smooth(self, L, x, b, nu):
+        for _ in range(nu):
+            x = x + self.omega * L["ir"] * (b - L["A"] @ x)
+        return x
+
+    def _coarse(self, L, b):
+        if L["dense"]:
+            bl = b.copy()
+            bl[0] = 0.0
+            return L["Ainv"] @ bl
+        bl = b.copy()
+        bl[0] = 0.0
+        return self._smooth(L, np.zeros_like(b), bl, self.coarse_sweeps)
+
+    def vcycle(self, b, x=None, lvl=0):
+        L = self.levels[lvl]
+        if x is None:
+            x = np.zeros_like(b)
+        if L["coarsest"]:
+            return self._coarse(L, b)
+        x = self._smooth(L, x, b, self.pre)
+        r = b - L["A"] @ x
+        ec = self.vcycle(L["R"] @ r, None, lvl + 1)
+        x = x + L["P"] @ ec
+        return self._smooth(L, x, b, self.post)
+
+    def precondition(self, v):
+        """Preconditioner of D^-1 A: z = S V(S D v), S = D^-1/2 or I."""
+        return self.dh * self.vcycle(self.dh * self.diag0 * v)
+
+    def apply(self, b, cycles=1, tol=5e-2, max_cycles=8):
+        """classical_amg.py:1157-1180 (equilibrated internally, physical in/out)."""
+        bs = self.dh * b
+        x = np.zeros_like(bs)
+        A = self.levels[0]["A"]
+        c = 0
+        while c < max_cycles:
+            x = self.vcycle(bs, x)
+            c += 1
+            rel = np.linalg.norm(bs - A @ x) / (np.linalg.norm(bs) + 1e-30)
+            if c >= cycles and rel < tol:
+                break
+        return self.dh * x
+
+
+def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, restart=30, maxit=300):
+    """Right-preconditioned restarted FGMRES with CGS2 on D^-1 A x = D^-1 b
+    (same algorithm as the HIP solver). Returns (x, stats)."""
+    A = sp.csr_matrix(A)
+    dinv = 1.0 / A.diagonal()
+    bs = dinv * b
+    bnorm = np.linalg.norm(bs)
+    x = np.zeros_like(b) if x0 is None else x0.astype(np.float64).copy()
+    if bnorm == 0:
+        return np.zeros_like(b), {"iterations": 0, "converged": True, "rel_residual": 0.0}
+    tol = rtol * bnorm
+    r = dinv * (b - A @ x)
+    beta = np.linalg.norm(r)
+    its = 0
+    m = restart
+    while beta > tol and its < maxit:
+        V = np.zeros((m + 1, b.size))
+        Z = np.zeros((m, b.size))
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        V[0] = r / beta
+        k = 0
+        for j in range(m):
+            Z[j] = precond(V[j]) if precond is not None else V[j]
+            w = dinv * (A @ Z[j])
+            h = V[:j + 1] @ w
+            w = w - V[:j + 1].T @ h
+            h2 = V[:j + 1] @ w
+            w = w - V[:j + 1].T @ h2
+            H[:j + 1, j] = h + h2
+            hn = np.linalg.norm(w)
+            H[j + 1, j] = hn
+            for i in range(j):
+                a, bb = H[i, j], H[i + 1, j]
+                H[i, j] = cs[i] * a + sn[i] * bb
+                H[i + 1, j] = -sn[i] * a + cs[i] * bb
+            a, bb = H[j, j], H[j + 1, j]
+            if bb == 0:
+                c, s = 1.0, 0.0
+            elif abs(bb) > abs(a):
+                t = a / bb
+                s = 1 / np.sqrt(1 + t * t)
+                c = s * t
+            else:
+                t = bb / a
+                c = 1 / np.sqrt(1 + t * t)
+                s = c * t
+            cs[j], sn[j] = c, s
+            H[j, j] = c * a + s * bb
+            H[j + 1, j] = 0.0
+            g[j + 1] = -s * g[j]
+            g[j] = c * g[j]
+            V[j + 1] = w / (hn if hn > 0 else 1.0)
+            its += 1
+            k = j + 1
+            if not abs(g[j + 1]) > tol or its >= maxit:
+                break
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i] if H[i, i] != 0 else 0.0
+        x = x + Z[:k].T @ y
+        r = dinv * (b - A @ x)
+        beta = np.linalg.norm(r)
+    return x, {"iterations": its, "converged": bool(beta <= tol), "rel_residual": beta / bnorm}
