@@ -1,0 +1,88 @@
+"""Pins the FP64 IMPES oracle (oracle/impes.py) against the reference's own
+functions (golden vectors from tests/golden/make_golden.py, float32) and
+against closed forms / invariants. CPU only."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import impes as oi
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(GOLD, "steps.json")))
+CASES = sorted(META)
+
+
+def load_case(name):
+    m = META[name]
+    pvt = oi.PVT(m["pvt"]) if m["pvt"] else None
+    case = oi.Case(m["config"], pvt=pvt)
+    return case, m, np.load(os.path.join(GOLD, f"step_{name}.npz"))
+
+
+def state_from(z, s, dims):
+    return {k: z[f"s{s}_before_{k}"].astype(np.float64).reshape(dims) for k in ("p", "sw", "sg")}
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pressure_system_matches_reference(name):
+    case, m, z = load_case(name)
+    dt = m["dt_days"] * 86400.0
+    n = int(np.prod(case.dims))
+    for s in range(len(m["steps"])):
+        st = state_from(z, s, case.dims)
+        ps = oi.pressure_system(case, st, dt)
+        A = oi.system_csr(case, ps)
+        Aref = sp.csr_matrix((z[f"s{s}_A_val"].astype(np.float64), (z[f"s{s}_A_row"], z[f"s{s}_A_col"])),
+                             shape=(n, n))
+        # the accumulation coefficient PV*ct/dt carries the reference's float32
+        # central difference of PVT tables (~1e-4 relative noise): diagonal and
+        # rhs get a looser bar when a PVT table is present; ct itself is pinned
+        # exactly in test_compressibility_float32_emulation
+        loose = 2e-3 if case.pvt is not None else 2e-5
+        D = (A - Aref).tocoo()
+        if D.nnz:
+            scale = abs(Aref).tocsr()[D.row, D.col].A1 + 1e-300
+            tol = np.where(D.row == D.col, loose, 2e-5)
+            assert np.all(np.abs(D.data) / scale < tol), f"{name} step {s}: matrix"
+        bref = z[f"s{s}_b"].astype(np.float64)
+        sc = np.abs(bref) + np.abs(ps["acc"] * st["p"].ravel())
+        assert np.max(np.abs(ps["b"] - bref) / (sc + 1e-300)) < loose, f"{name} step {s}: rhs"
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_saturation_step_matches_reference(name):
+    case, m, z = load_case(name)
+    dt = m["dt_days"] * 86400.0
+    prev = None
+    for s, ref in enumerate(m["steps"]):
+        st = state_from(z, s, case.dims)
+        ps = oi.pressure_system(case, st, dt)
+        pnew = z[f"s{s}_p_new"].astype(np.float64).reshape(case.dims)
+        # the reference's literal BHP form on its own float32 pressure
+        out = oi.saturation_step(case, st, pnew, dt, ps, bhp_form="direct")
+        assert out["N"] == ref["N"], f"{name} step {s}: substeps"
+        sw_ref = z[f"s{s}_sw"].astype(np.float64).reshape(case.dims)
+        sg_ref = z[f"s{s}_sg"].astype(np.float64).reshape(case.dims)
+        assert np.max(np.abs(out["sw"] - sw_ref)) < 5e-6, f"{name} step {s}: Sw"
+        assert np.max(np.abs(out["sg"] - sg_ref)) < 5e-6, f"{name} step {s}: Sg"
+        # per-step ledger increments
+        for ph in ("oil", "gas"):
+            for io_ in ("in", "out"):
+                inc = ref["comp"][ph][io_] - (prev["comp"][ph][io_] if prev else 0.0)
+                got = out["comp"][f"{ph}_{io_}"]
+                assert math.isclose(got, inc, rel_tol=2e-4, abs_tol=1e-6 * (1 + abs(inc))), (name, s, ph, io_)
+        for ph in ("water", "oil", "gas"):
+            for io_ in ("in", "out"):
+                inc = ref["mass"][ph][io_] - (prev["mass"][ph][io_] if prev else 0.0)
+                got = float(out["mass"][f"{ph}_{io_}"])
+                assert math.isclose(got, inc, rel_tol=2e-3, abs_tol=1e-3 * (1 + abs(inc))), (name, s, ph, io_)
+        for wd, wr in zip(out["wells"], ref["wells"]):
+            if wr["last_q_total"] is not None and wd["last_q_total"] is not None:
+                assert math.isclose(wd["last_q_total"], wr["last_q_total"], rel_tol=1e-4), (name, s)
+            if wd["last_mode"] is not None or wr["last_mode"] is not None:
+                assert wd["last_mode"] == wr["last_mode"]
+        prev = ref
