@@ -55,6 +55,10 @@ int impes_l1jacobi_csr_f32(int n, int nnz, const int* rowptr, const int* col, co
                            const float* inv_relax, float omega, const float* x_in, const float* b, float* x_out,
                            void* stream);
 
+/* diag[i] = a_ii (0 if the row has no diagonal entry); used to form the row scaling
+ * D^-1 of the Krylov solver. Replaces ClassicalAMG._extract_diag, classical_amg.py:940-956. */
+int impes_csr_diagonal(int n, const int* rowptr, const int* col, const double* val, double* diag, void* stream);
+
 /* out[0] = x . y (device scalar, FP64 accumulation, deterministic two-stage
  * reduction). Replaces the torch.dot / norm calls of the solvers,
  * src/solver/classical_amg.py:1136-1142, src/simulator/simulation.py:1677-1689. */
@@ -138,6 +142,96 @@ int impes_fgmres_st7(impes_solver* solver, impes_amg* amg, int precision, int nx
                      const double* d, const double* ox, const double* oy, const double* oz, const double* dinv,
                      const double* b, double* x, int use_x0, double rtol, int restart, int maxit,
                      impes_solve_stats* stats, void* stream);
+
+/* Stationary V-cycle iteration in physical space (x = D^-1/2-rescaled cycles of the
+ * equilibrated matrix), stopping on ||bs - A xs||/||bs|| < tol after >= min_cycles.
+ * Replaces ClassicalAMG.solve (classical_amg.py:1108-1155, use_x0 = x0 given,
+ * min_cycles = 1) and ClassicalAMG.apply (classical_amg.py:1157-1180). rel and
+ * cycles_done are host pointers (may be NULL). */
+int impes_amg_cycles(impes_amg* amg, int precision, const double* b, double* x, int use_x0, int min_cycles,
+                     double tol, int max_cycles, double* rel, int* cycles_done, void* stream);
+
+/* ------------------------------------------------------------------ IMPES model */
+
+typedef struct {
+  int nx, ny, nz;     /* Reservoir.dimensions, reservoir.py:18-19 */
+  double dx, dy, dz;  /* Reservoir.grid_size, reservoir.py:22-23 */
+} impes_grid;
+
+/* Fluid.__init__ parameters (fluid.py:26-110) and scheme knobs (simulation.py:826-828,1058,1354,1377). */
+typedef struct {
+  int stone2;                          /* relative_permeability.model == "stone2" */
+  double nw, no, ng, ko_end_w, ko_end_g, now_, nog;
+  double sw_cr, so_r, sg_cr;
+  double mu_o, mu_w, mu_g;             /* Pa s, used without a PVT table */
+  double rho_o_ref, rho_w_ref, rho_g_ref;
+  double c_o, c_w, c_g, c_r, cf;       /* 1/Pa; cf = (c_o+c_w+c_g+c_r)/3 */
+  double pc_scale, pc_exp, pcog_scale, pcog_exp;
+  double g;                            /* 9.81 */
+  double max_ds;                       /* max_saturation_change */
+  int max_substeps;
+  double rate_scale;                   /* global_rate_scale */
+} impes_fluid;
+
+/* One well (well.py:10-55); cell is the z-fastest flat index (simulation.py:816). */
+typedef struct {
+  int cell;
+  int producer;       /* 0 injector, 1 producer */
+  int control;        /* 0 rate (m3/day), 1 bhp (MPa), 2 ignored */
+  int gas;            /* injected phase: 0 water, 1 gas */
+  int surface;        /* rate_type == "surface" */
+  int surface_phase;  /* 0 oil, 1 water, 2 gas, 3 liquid */
+  int has_bhp_min;
+  int lim_mask;       /* bit0 wopr, bit1 wlpr, bit2 wgpr, bit3 liqr */
+  double value;
+  double wi;          /* Peaceman index of well.py:44-51 */
+  double bhp_min;     /* MPa */
+  double lim[4];      /* m3/day */
+} impes_well;
+
+/* Creates the device model of one reservoir. pvt_host (host) holds 9*pvt_n doubles:
+ * pressure_MPa, Bo, Bw, Bg, mu_o_cP, mu_w_cP, mu_g_cP, Rs, Rv (pvt.py:18-46); pvt_n = 0: no table.
+ * Takes over Simulator.__init__ (simulation.py:19-106) for the IMPES branch. */
+int impes_model_create(const impes_grid* grid, const impes_fluid* fluid, int pvt_n, const double* pvt_host,
+                       int nwells, const impes_well* wells_host, impes_model** out);
+int impes_model_destroy(impes_model* m);
+/* kx, ky, kz in m^2 and pore volume in m^3 per cell (device): transmissibilities of
+ * simulation.py:52-74 computed on the device. */
+int impes_model_set_rock(impes_model* m, const double* kx, const double* ky, const double* kz, const double* pv,
+                         void* stream);
+int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host);
+int impes_model_set_amg(impes_model* m, const impes_amg_params* params);
+int impes_model_report_len(const impes_model* m, int* out);
+/* Pressure equation at (p, sw, sg): Simulator._impes_pressure_step up to the solve
+ * (simulation.py:867-902): mobilities, upwinding, _calculate_well_terms,
+ * _build_pressure_matrix_vectorized, _build_pressure_rhs. */
+int impes_model_assemble(impes_model* m, double dt, const double* p, const double* sw, const double* sg,
+                         void* stream);
+/* AMG-preconditioned FP64 FGMRES on the assembled system (AMG setup on the first call,
+ * numeric update after); p_prev is the initial guess, p_new the solution.
+ * Replaces the "amg" branch of simulation.py:904-929. */
+int impes_model_solve_pressure(impes_model* m, int precision, double rtol, int restart, int maxit,
+                               const double* p_prev, double* p_new, impes_solve_stats* stats, void* stream);
+/* Jacobi-preconditioned CG, reference semantics (absolute stop on sqrt(r.z), x0 = 0):
+ * Simulator._solve_pressure_cg_pytorch, simulation.py:1662-1697. converged/iterations: host. */
+int impes_model_solve_cg(impes_model* m, double tol, int maxit, double* p_new, int* converged, int* iterations,
+                         void* stream);
+/* Explicit transport: Simulator._impes_saturation_step (simulation.py:968-1544).
+ * report_host receives impes_model_report_len doubles (substeps, ledgers, per-well diagnostics). */
+int impes_model_saturation(impes_model* m, double dt, const double* p_prev, const double* p_new, const double* sw,
+                           const double* sg, double* sw_out, double* sg_out, double* so_out, double* report_host,
+                           void* stream);
+/* Copies a model field into out (device, n cells): 0 diag, 1 ox, 2 oy, 3 oz, 4 rhs, 5 accumulation,
+ * 6 gravity+capillary rhs, 7 tx, 8 ty, 9 tz, 10 pore volume, 11 total mobility, 12 1/diag. */
+int impes_model_export(impes_model* m, int field, double* out, void* stream);
+/* Elementwise fluid properties (fluid.py:129-248,273-296,330-435): out is 12 x n:
+ * kro krw krg pc_ow pc_og ct rho_w rho_o rho_g mu_o mu_w mu_g (Pa s). */
+int impes_model_eval_props(impes_model* m, long long n, const double* p, const double* sw, const double* sg,
+                           double* out, void* stream);
+/* PVT lookup (pvt.py:52-64 / fluid.py:182-188), key 0..7 = Bo Bw Bg mu_o mu_w mu_g Rs Rv. */
+int impes_model_eval_pvt(impes_model* m, int key, long long n, const double* p, double* out, void* stream);
+/* Hierarchy summary of the pressure solver (after the first solve). */
+int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_host, int* nnz_host, int max_levels);
 
 #ifdef __cplusplus
 }

@@ -33,7 +33,7 @@ struct LevelInfo {
 
 struct Level {
   int n = 0, nnz = 0, group = 1;
-  bool coarsest = false, dense = false;
+  bool coarsest = false, dense = false, pin = true;
   DevBuf<int> rp, ci;
   DevBuf<double> v;
   DevBuf<float> v32;
@@ -81,6 +81,13 @@ class Hierarchy {
   // z = S .* V(S .* D .* v)  with S = D^-1/2 (equilibrated) or I.
   void precondition(int precision, const double* v, double* z, hipStream_t s);
 
+  // Stationary V-cycle iteration with physical-space input/output, the semantics of
+  // ClassicalAMG.solve (classical_amg.py:1108-1155) and .apply (:1157-1180):
+  // bs = dh.*b, xs = x0./dh (or 0); cycle until ||bs - A0 xs||/||bs|| < tol after at
+  // least min_cycles, or max_cycles; x = dh.*xs. Returns the cycles done, *rel the residual.
+  int cycles(int precision, const double* b, double* x, bool use_x0, int min_cycles, double tol, int max_cycles,
+             double* rel, hipStream_t s);
+
   int n0() const { return levels_.empty() ? 0 : levels_[0].n; }
   int num_levels() const { return (int)levels_.size(); }
   LevelInfo info(int l) const;
@@ -119,8 +126,7 @@ class Hierarchy {
   DevBuf<double> st_d_, st_x_, st_y_, st_z_;  // unscaled, as given
   DevBuf<double> sd_, sx_, sy_, sz_;          // equilibrated
   DevBuf<float> sd32_, sx32_, sy32_, sz32_;
-  DevBuf<double> pre_b64_, pre_x64_;
-  DevBuf<float> pre_b32_, pre_x32_;
+  DevBuf<double> cyc_b_, cyc_x_, cyc_r_, cyc_s_;
 };
 
 // stencil helpers (amg.hip)

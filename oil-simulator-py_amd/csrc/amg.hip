@@ -128,8 +128,8 @@ struct Sel<double> {
   }
   static void copy(double* y, const double* x, long long n, hipStream_t s) { launch_copy_f64(y, x, n, s); }
   static void pin0(double* b, hipStream_t s) { k_set_pinned_zero<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
-  static void dense(int n, const double* aug, const double* b, double* x, hipStream_t s) {
-    launch_dense_solve_f64(n, aug, b, x, s);
+  static void dense(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s) {
+    launch_dense_solve_f64(n, pin, aug, b, x, s);
   }
 };
 template <>
@@ -156,8 +156,8 @@ struct Sel<float> {
   }
   static void copy(float* y, const float* x, long long n, hipStream_t s) { launch_copy_f32(y, x, n, s); }
   static void pin0(float* b, hipStream_t s) { k_set_pinned_zero32<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
-  static void dense(int n, const double* aug, const float* b, float* x, hipStream_t s) {
-    launch_dense_solve_f32(n, aug, b, x, s);
+  static void dense(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s) {
+    launch_dense_solve_f32(n, pin, aug, b, x, s);
   }
 };
 
@@ -409,9 +409,10 @@ void Hierarchy::finish_level_values(int l, hipStream_t s) {
 void Hierarchy::build_coarsest(int l, hipStream_t s) {
   Level& L = levels_[l];
   L.dense = L.n <= prm_.dense_max;
+  L.pin = levels_.size() > 1;
   if (!L.dense) return;
   L.aug.alloc(2LL * L.n * L.n);
-  launch_csr_to_dense_pinned(L.n, L.rp.p, L.ci.p, L.v.p, L.aug.p, s);
+  launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin ? 1 : 0, L.aug.p, s);
   launch_gauss_jordan_inverse(L.n, L.aug.p, s);
 }
 
@@ -462,7 +463,7 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
     launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
     launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
     if (L.coarsest && L.dense) {
-      launch_csr_to_dense_pinned(L.n, L.rp.p, L.ci.p, L.v.p, L.aug.p, s);
+      launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin ? 1 : 0, L.aug.p, s);
       launch_gauss_jordan_inverse(L.n, L.aug.p, s);
     }
   }
@@ -513,13 +514,13 @@ template <class T>
 void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
   Level& L = levels_[l];
   if (L.dense) {
-    Sel<T>::dense(L.n, L.aug.p, b, x, s);
+    Sel<T>::dense(L.n, L.pin ? 1 : 0, L.aug.p, b, x, s);
     return;
   }
   // reference: 50 L1-Jacobi sweeps from zero on the unpinned matrix with b_0 = 0
   T* bl = Sel<T>::r(L);
   Sel<T>::copy(bl, b, L.n, s);
-  Sel<T>::pin0(bl, s);
+  if (L.pin) Sel<T>::pin0(bl, s);
   T* cur = x;
   T* other = Sel<T>::t(L);
   bool zero = true;
@@ -575,6 +576,58 @@ void Hierarchy::vcycle64(const double* b, double* x, bool x_zero, hipStream_t s)
 void Hierarchy::vcycle32(const float* b, float* x, bool x_zero, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "vcycle before setup");
   vcycle_level<float>(0, b, x, x_zero, s);
+}
+
+int Hierarchy::cycles(int precision, const double* b, double* x, bool use_x0, int min_cycles, double tol,
+                      int max_cycles, double* rel, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "cycles before setup");
+  Level& L0 = levels_[0];
+  const long long n = L0.n;
+  cyc_b_.reserve(n); cyc_x_.reserve(n); cyc_r_.reserve(n);
+  cyc_s_.reserve(kRedBlocks + 4);
+  double* work = cyc_s_.p;
+  double* sc = cyc_s_.p + kRedBlocks;
+  double hv[2];
+  launch_scale_f64(cyc_b_.p, dh_.p, b, n, s);  // bs = dh .* b
+  if (use_x0) launch_div_f64(cyc_x_.p, x, dh_.p, n, s);  // xs = x0 ./ dh
+  launch_dot_f64(cyc_b_.p, cyc_b_.p, n, sc, work, s);
+  IMPES_HIP_CHECK(hipMemcpyAsync(hv, sc, sizeof(double), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(hv[0]);
+  bool zero = !use_x0;
+  int c = 0;
+  double r = INFINITY;
+  while (c < max_cycles) {
+    if (precision == 32) {
+      launch_f64_to_f32(L0.b32.p, cyc_b_.p, n, s);
+      if (!zero) launch_f64_to_f32(L0.x32.p, cyc_x_.p, n, s);
+      vcycle_level<float>(0, L0.b32.p, L0.x32.p, zero, s);
+      DevBuf<double> ones;
+      ones.alloc(n);
+      launch_fill_f64(ones.p, 1.0, n, s);
+      launch_scale_f32_to_f64(cyc_x_.p, ones.p, L0.x32.p, n, s);
+      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      vcycle_level<double>(0, cyc_b_.p, cyc_x_.p, zero, s);
+    }
+    zero = false;
+    ++c;
+    if (stencil_) {
+      Stencil7<double> S{nx_, ny_, nz_, sd_.p, sx_.p, sy_.p, sz_.p};
+      launch_st7_residual_f64(S, cyc_x_.p, cyc_b_.p, cyc_r_.p, s);
+    } else {
+      launch_residual_f64(Sel<double>::A(L0), cyc_x_.p, cyc_b_.p, cyc_r_.p, L0.group, s);
+    }
+    launch_dot_f64(cyc_r_.p, cyc_r_.p, n, sc + 1, work, s);
+    IMPES_HIP_CHECK(hipMemcpyAsync(hv + 1, sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    r = std::sqrt(hv[1]) / (bnorm + 1e-30);
+    if (c >= min_cycles && r < tol) break;
+  }
+  launch_scale_f64(x, dh_.p, cyc_x_.p, n, s);
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  if (rel) *rel = r;
+  return c;
 }
 
 void Hierarchy::precondition(int precision, const double* v, double* z, hipStream_t s) {

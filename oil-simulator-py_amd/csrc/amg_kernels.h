@@ -68,11 +68,11 @@ void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mr
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
                            const double* mv, double* cv, hipStream_t s);
 
-// coarsest level (coarse.hip): dense inverse of the DOF-0-pinned matrix
-void launch_csr_to_dense_pinned(int n, const int* rp, const int* ci, const double* v, double* dense, hipStream_t s);
+// coarsest level (coarse.hip): dense inverse of the (optionally DOF-0-pinned) matrix
+void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, int pin, double* dense, hipStream_t s);
 void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s);  // aug is n x 2n row-major, inverse in right half
-// x = Ainv * b with b[0] treated as 0 (pinned DOF); FP64 and FP32 vector variants
-void launch_dense_solve_f64(int n, const double* aug, const double* b, double* x, hipStream_t s);
-void launch_dense_solve_f32(int n, const double* aug, const float* b, float* x, hipStream_t s);
+// x = Ainv * b (with pin: b[0] treated as 0); FP64 and FP32 vector variants
+void launch_dense_solve_f64(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s);
+void launch_dense_solve_f32(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s);
 
 }  // namespace impes

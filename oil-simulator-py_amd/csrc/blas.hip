@@ -109,6 +109,17 @@ __global__ void k_lincomb_add(double* __restrict__ x, const double* __restrict__
   }
 }
 
+__global__ void k_div(double* __restrict__ y, const double* __restrict__ x, const double* __restrict__ a,
+                      long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i] / a[i];
+}
+
+__global__ void k_xpby(double* __restrict__ y, const double* __restrict__ x, double beta, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i] + beta * y[i];
+}
+
 // v may alias w (in-place normalisation)
 __global__ void k_scale_by_inv(double* v, const double* w, const double* __restrict__ nrm,
                                long long n) {
@@ -167,6 +178,16 @@ void launch_scale_f64(double* y, const double* a, const double* x, long long n, 
 void launch_axpy_f64(double* y, double alpha, const double* x, long long n, hipStream_t s) {
   if (n <= 0) return;
   k_axpy<<<grid_for(n, kBlock), kBlock, 0, s>>>(y, alpha, x, n);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_div_f64(double* y, const double* x, const double* a, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_div<<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, a, n);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_xpby<<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, beta, n);
   IMPES_LAUNCH_CHECK();
 }
 void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s) {

@@ -1,10 +1,14 @@
 // extern "C" entry points: argument checking, exception -> error code mapping.
+#include <cstddef>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/impes_hip.h"
 #include "amg.h"
 #include "amg_kernels.h"
 #include "fgmres.h"
+#include "impes_model.h"
 
 namespace impes {
 namespace {
@@ -86,6 +90,13 @@ int impes_l1jacobi_csr_f32(int n, int nnz, const int* rowptr, const int* col, co
     IMPES_REQUIRE(x_in != x_out, "x_in and x_out must differ (Jacobi sweep)");
     CsrF32 A{n, n, nnz, rowptr, col, val};
     launch_jacobi_f32(A, x_in, b, inv_relax, omega, x_out, csr_group_for(n, nnz), as_stream(stream));
+  });
+}
+
+int impes_csr_diagonal(int n, const int* rowptr, const int* col, const double* val, double* diag, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(n >= 0 && rowptr && col && val && diag, "null pointer");
+    launch_diag_relax(n, rowptr, col, val, diag, nullptr, nullptr, as_stream(stream));
   });
 }
 
@@ -343,6 +354,191 @@ int impes_fgmres_st7(impes_solver* solver, impes_amg* amg, int precision, int nx
     solver->f.solve(A, amg ? &amg->h : nullptr, precision, b, x, use_x0 != 0, rtol, restart, maxit, st,
                     as_stream(stream));
     fill_stats(stats, st);
+  });
+}
+
+int impes_amg_cycles(impes_amg* amg, int precision, const double* b, double* x, int use_x0, int min_cycles,
+                     double tol, int max_cycles, double* rel, int* cycles_done, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(amg && b && x, "null pointer");
+    IMPES_REQUIRE(amg->h.num_levels() > 0, "not set up");
+    IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    IMPES_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+    const int c = amg->h.cycles(precision, b, x, use_x0 != 0, min_cycles, tol, max_cycles, rel, as_stream(stream));
+    if (cycles_done) *cycles_done = c;
+  });
+}
+
+// ------------------------------------------------------------------ model
+static_assert(sizeof(impes_fluid) == sizeof(FluidParams), "impes_fluid layout");
+static_assert(offsetof(impes_fluid, rate_scale) == offsetof(FluidParams, rate_scale), "impes_fluid layout");
+static_assert(offsetof(impes_fluid, max_substeps) == offsetof(FluidParams, max_substeps), "impes_fluid layout");
+static_assert(sizeof(impes_well) == sizeof(WellDesc), "impes_well layout");
+static_assert(offsetof(impes_well, lim) == offsetof(WellDesc, lim), "impes_well layout");
+static_assert(offsetof(impes_well, value) == offsetof(WellDesc, value), "impes_well layout");
+
+struct impes_model {
+  Model m;
+  impes_model(const Grid& g, const FluidParams& f, int pn, const double* ph, const std::vector<WellDesc>& w)
+      : m(g, f, pn, ph, w) {}
+};
+
+static std::vector<WellDesc> wells_from(int nwells, const impes_well* w) {
+  IMPES_REQUIRE(nwells >= 0, "negative well count");
+  IMPES_REQUIRE(nwells == 0 || w, "null well array");
+  std::vector<WellDesc> out(nwells);
+  if (nwells) std::memcpy(out.data(), w, sizeof(WellDesc) * nwells);
+  for (const auto& d : out) {
+    IMPES_REQUIRE(d.producer == 0 || d.producer == 1, "well: producer flag must be 0/1");
+    IMPES_REQUIRE(d.control >= 0 && d.control <= 2, "well: bad control");
+    IMPES_REQUIRE(d.surface_phase >= 0 && d.surface_phase <= 3, "well: bad surface phase");
+  }
+  return out;
+}
+
+int impes_model_create(const impes_grid* grid, const impes_fluid* fluid, int pvt_n, const double* pvt_host,
+                       int nwells, const impes_well* wells_host, impes_model** out) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(grid && fluid && out, "null pointer");
+    IMPES_REQUIRE(pvt_n >= 0, "negative PVT length");
+    Grid g{grid->nx, grid->ny, grid->nz, grid->dx, grid->dy, grid->dz};
+    FluidParams f;
+    std::memcpy(&f, fluid, sizeof(f));
+    IMPES_REQUIRE(f.max_substeps >= 1, "max_substeps must be >= 1");
+    IMPES_REQUIRE(f.max_ds > 0.0, "max_saturation_change must be > 0");
+    *out = new impes_model(g, f, pvt_n, pvt_host, wells_from(nwells, wells_host));
+  });
+}
+
+int impes_model_destroy(impes_model* m) {
+  return guarded(__func__, [&] { delete m; });
+}
+
+int impes_model_set_rock(impes_model* m, const double* kx, const double* ky, const double* kz, const double* pv,
+                         void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && kx && ky && kz && pv, "null pointer");
+    m->m.set_rock(kx, ky, kz, pv, as_stream(stream));
+  });
+}
+
+int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m, "null model");
+    m->m.set_wells(wells_from(nwells, wells_host));
+  });
+}
+
+int impes_model_set_amg(impes_model* m, const impes_amg_params* params) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && params, "null pointer");
+    AmgParams p;
+    p.theta = params->theta;
+    p.max_levels = params->max_levels;
+    p.coarsest_size = params->coarsest_size;
+    p.pre_smooth = params->pre_smooth;
+    p.post_smooth = params->post_smooth;
+    p.omega = params->omega;
+    p.equil_threshold = params->equil_threshold;
+    p.dense_max = params->dense_max;
+    p.coarse_sweeps = params->coarse_sweeps;
+    p.anchor = params->anchor;
+    IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
+    IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= 512, "dense_max must be in [1, 512]");
+    m->m.set_amg_params(p);
+  });
+}
+
+int impes_model_report_len(const impes_model* m, int* out) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && out, "null pointer");
+    *out = m->m.report_len();
+  });
+}
+
+int impes_model_assemble(impes_model* m, double dt, const double* p, const double* sw, const double* sg,
+                         void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p && sw && sg, "null pointer");
+    m->m.assemble(dt, p, sw, sg, as_stream(stream));
+  });
+}
+
+int impes_model_solve_pressure(impes_model* m, int precision, double rtol, int restart, int maxit,
+                               const double* p_prev, double* p_new, impes_solve_stats* stats, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p_prev && p_new, "null pointer");
+    IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    IMPES_REQUIRE(rtol > 0.0 && maxit >= 1, "bad tolerance / iteration limit");
+    hipStream_t s = as_stream(stream);
+    if (p_new != p_prev)
+      IMPES_HIP_CHECK(hipMemcpyAsync(p_new, p_prev, sizeof(double) * m->m.n(), hipMemcpyDeviceToDevice, s));
+    SolveStats st;
+    m->m.solve(precision, rtol, restart, maxit, p_new, st, s);
+    fill_stats(stats, st);
+  });
+}
+
+int impes_model_solve_cg(impes_model* m, double tol, int maxit, double* p_new, int* converged, int* iterations,
+                         void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p_new, "null pointer");
+    IMPES_REQUIRE(maxit >= 0, "bad iteration limit");
+    int its = 0;
+    const bool ok = m->m.solve_cg(tol, maxit, p_new, its, as_stream(stream));
+    if (converged) *converged = ok ? 1 : 0;
+    if (iterations) *iterations = its;
+  });
+}
+
+int impes_model_saturation(impes_model* m, double dt, const double* p_prev, const double* p_new, const double* sw,
+                           const double* sg, double* sw_out, double* sg_out, double* so_out, double* report_host,
+                           void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p_prev && p_new && sw && sg && sw_out && sg_out && report_host, "null pointer");
+    IMPES_REQUIRE(sw_out != sw && sg_out != sg, "saturation outputs must not alias the inputs");
+    IMPES_REQUIRE(dt > 0.0, "dt must be positive");
+    m->m.saturation(dt, p_prev, p_new, sw, sg, sw_out, sg_out, so_out, report_host, as_stream(stream));
+  });
+}
+
+int impes_model_export(impes_model* m, int field, double* out, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && out, "null pointer");
+    Model& M = m->m;
+    const double* src[] = {M.diag.p, M.ox.p, M.oy.p, M.oz.p, M.b.p, M.acc.p, M.extra.p,
+                           M.tx.p, M.ty.p, M.tz.p, M.pv.p, M.mob_t.p, M.dinv.p};
+    IMPES_REQUIRE(field >= 0 && field < (int)(sizeof(src) / sizeof(src[0])), "bad field id");
+    IMPES_HIP_CHECK(hipMemcpyAsync(out, src[field], sizeof(double) * M.n(), hipMemcpyDeviceToDevice,
+                                   as_stream(stream)));
+  });
+}
+
+int impes_model_eval_props(impes_model* m, long long n, const double* p, const double* sw, const double* sg,
+                           double* out, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p && sw && sg && out, "null pointer");
+    m->m.eval_props(n, p, sw, sg, out, as_stream(stream));
+  });
+}
+
+int impes_model_eval_pvt(impes_model* m, int key, long long n, const double* p, double* out, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p && out, "null pointer");
+    m->m.eval_pvt(key, n, p, out, as_stream(stream));
+  });
+}
+
+int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_host, int* nnz_host, int max_levels) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && num_levels, "null pointer");
+    const Hierarchy* h = m->m.hierarchy();
+    *num_levels = h ? h->num_levels() : 0;
+    for (int l = 0; h && l < h->num_levels() && l < max_levels; ++l) {
+      LevelInfo li = h->info(l);
+      if (sizes_host) sizes_host[l] = li.n;
+      if (nnz_host) nnz_host[l] = li.nnz;
+    }
   });
 }
 

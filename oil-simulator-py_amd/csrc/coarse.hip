@@ -1,6 +1,8 @@
 // Coarsest-level direct solve of the V-cycle (classical_amg.py:1045-1069):
 // the level matrix is made dense, DOF 0 is pinned (row/column 0 zeroed, a_00 = 1,
-// b_0 = 0), and the system is solved exactly. The inverse is formed once per
+// b_0 = 0) when the hierarchy has more than one level, and the system is solved
+// exactly. A single-level hierarchy is solved without the pin: the pin exists for
+// singular (pure Neumann) matrices, and on its own it would freeze DOF 0. The inverse is formed once per
 // hierarchy update by Gauss-Jordan elimination with partial pivoting in FP64, so
 // each V-cycle visit is a single dense matrix-vector product.
 #include "amg_kernels.h"
@@ -9,8 +11,7 @@ namespace impes {
 
 namespace {
 
-__global__ void k_csr_to_dense_pinned(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                      const double* __restrict__ v, double* __restrict__ aug) {
+__global__ void k_dense_identity(int n, double* __restrict__ aug) {
   // aug is n x 2n: [A_pinned | I]
   const long long w = 2LL * n;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < (long long)n * w;
@@ -21,15 +22,15 @@ __global__ void k_csr_to_dense_pinned(int n, const int* __restrict__ rp, const i
 }
 
 __global__ void k_dense_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                const double* __restrict__ v, double* __restrict__ aug) {
+                                const double* __restrict__ v, int pin, double* __restrict__ aug) {
   const long long w = 2LL * n;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
     for (int e = rp[r]; e < rp[r + 1]; ++e) {
       const int c = ci[e];
-      if (r == 0 || c == 0) continue;  // pinned DOF 0
+      if (pin && (r == 0 || c == 0)) continue;  // pinned DOF 0
       aug[r * w + c] += v[e];
     }
-    if (r == 0) aug[0] = 1.0;
+    if (pin && r == 0) aug[0] = 1.0;
   }
 }
 
@@ -92,15 +93,15 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_dense_solve(int n, const double* __restrict__ aug, const T* __restrict__ b,
-                                                     T* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_dense_solve(int n, int pin, const double* __restrict__ aug,
+                                                     const T* __restrict__ b, T* __restrict__ x) {
   const long long w = 2LL * n;
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int nwaves = (gridDim.x * blockDim.x) / kWave;
   for (int r = wid; r < n; r += nwaves) {
     double s = 0.0;
-    for (int c = 1 + lane; c < n; c += kWave) s += aug[(long long)r * w + n + c] * (double)b[c];
+    for (int c = (pin ? 1 : 0) + lane; c < n; c += kWave) s += aug[(long long)r * w + n + c] * (double)b[c];
     s = wave_sum(s);
     if (lane == 0) x[r] = (T)s;
   }
@@ -108,12 +109,12 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double* __rest
 
 }  // namespace
 
-void launch_csr_to_dense_pinned(int n, const int* rp, const int* ci, const double* v, double* aug, hipStream_t s) {
+void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, int pin, double* aug, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
   const long long tot = 2LL * n * n;
-  k_csr_to_dense_pinned<<<grid_for(tot, 256), 256, 0, s>>>(n, rp, ci, v, aug);
+  k_dense_identity<<<grid_for(tot, 256), 256, 0, s>>>(n, aug);
   IMPES_LAUNCH_CHECK();
-  k_dense_scatter<<<grid_for(n, 256), 256, 0, s>>>(n, rp, ci, v, aug);
+  k_dense_scatter<<<grid_for(n, 256), 256, 0, s>>>(n, rp, ci, v, pin, aug);
   IMPES_LAUNCH_CHECK();
 }
 
@@ -123,13 +124,13 @@ void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s) {
   IMPES_LAUNCH_CHECK();
 }
 
-void launch_dense_solve_f64(int n, const double* aug, const double* b, double* x, hipStream_t s) {
-  k_dense_solve<double><<<div_up(n, 4), 256, 0, s>>>(n, aug, b, x);
+void launch_dense_solve_f64(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s) {
+  k_dense_solve<double><<<div_up(n, 4), 256, 0, s>>>(n, pin, aug, b, x);
   IMPES_LAUNCH_CHECK();
 }
 
-void launch_dense_solve_f32(int n, const double* aug, const float* b, float* x, hipStream_t s) {
-  k_dense_solve<float><<<div_up(n, 4), 256, 0, s>>>(n, aug, b, x);
+void launch_dense_solve_f32(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s) {
+  k_dense_solve<float><<<div_up(n, 4), 256, 0, s>>>(n, pin, aug, b, x);
   IMPES_LAUNCH_CHECK();
 }
 

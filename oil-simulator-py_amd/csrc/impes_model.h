@@ -1,0 +1,92 @@
+// IMPES time step on the device: pressure assembly (face form), well terms, the
+// AMG-preconditioned FGMRES pressure solve, and the explicit upwind transport
+// with availability chokes, surface-rate conversions, limits, substeps and
+// ledgers. Restates Simulator._impes_* of src/simulator/simulation.py:822-1660.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "amg.h"
+#include "fgmres.h"
+#include "impes_phys.h"
+
+namespace impes {
+
+// report layout (doubles), one device->host copy per step
+enum ReportSlot {
+  kRatMax = 0, kNSub = 1,
+  kMassWIn = 2, kMassWOut, kMassOIn, kMassOOut, kMassGIn, kMassGOut,
+  kCompOIn = 8, kCompOOut, kCompGIn, kCompGOut,
+  kVolDW = 12, kVolDG, kVolQW, kVolQG, kVolCW, kVolCG,
+  kReportHead = 32,
+};
+// per-well record inside the report, after the head
+enum WellSlot {
+  kWMode = 0,      // -1 untouched, 0 rate, 1 bhp (pressure step)
+  kWTerm = 1,      // WI*lambda_t of the pressure equation (0 for rate wells)
+  kWQRate = 2,     // rate term of the pressure equation (m3/s)
+  kWHasQ = 3,      // 1 if last_q_total / surface rates were set this step
+  kWQTotal = 4,
+  kWSurf = 5,      // 4 surface rates: oil, water, gas, liquid (m3/day)
+  kWBal = 9,       // row-balance rate of a BHP well (m3/s), NaN if the direct form is used
+  kWellSlots = 12,
+};
+
+struct Grid {
+  int nx, ny, nz;
+  double dx, dy, dz;
+};
+
+class Model {
+ public:
+  Model(const Grid& g, const FluidParams& f, int pvt_n, const double* pvt_host, const std::vector<WellDesc>& wells);
+
+  // kx/ky/kz in m^2, pore volume per cell in m^3 (reservoir.py:50)
+  void set_rock(const double* kx, const double* ky, const double* kz, const double* pv, hipStream_t s);
+  void set_wells(const std::vector<WellDesc>& wells);
+  void set_amg_params(const AmgParams& p) { amg_prm_ = p; amg_.reset(); }
+  void reset_hierarchy() { amg_.reset(); }
+
+  // Pressure equation at the current state: face coefficients, diagonal, rhs, 1/diag.
+  void assemble(double dt, const double* p, const double* sw, const double* sg, hipStream_t s);
+  // AMG setup (first call) or numeric update, then FGMRES. p_new holds the initial guess on entry.
+  void solve(int precision, double rtol, int restart, int maxit, double* p_new, SolveStats& st, hipStream_t s);
+  // Jacobi-preconditioned CG with the reference's absolute stop sqrt(r.z) < tol from x = 0
+  // (simulation.py:1662-1697); returns converged.
+  bool solve_cg(double tol, int maxit, double* p_new, int& iterations, hipStream_t s);
+  // Explicit transport; fills the host report (head + wells) and synchronises.
+  void saturation(double dt, const double* p_prev, const double* p_new, const double* sw, const double* sg,
+                  double* sw_out, double* sg_out, double* so_out, double* host_report, hipStream_t s);
+  // Elementwise properties for API use: out is 12 x n (kro krw krg pc pcog ct rho_w rho_o rho_g mu_o mu_w mu_g)
+  void eval_props(long long n, const double* p, const double* sw, const double* sg, double* out, hipStream_t s);
+  void eval_pvt(int key, long long n, const double* p, double* out, hipStream_t s);
+
+  long long n() const { return N_; }
+  int num_wells() const { return nw_; }
+  int report_len() const { return kReportHead + kWellSlots * nw_; }
+  const Hierarchy* hierarchy() const { return amg_.get(); }
+
+  Grid grid;
+  FluidParams F;
+  PvtDev pvt;
+  DevBuf<double> tx, ty, tz, pv;                      // face form (0 on faces leaving the grid)
+  DevBuf<double> diag, ox, oy, oz, b, dinv, acc, extra, mob_t;
+
+ private:
+  long long N_;
+  int nw_ = 0;
+  bool gravity_ = false;
+  DevBuf<double> pvt_P_, pvt_col_, pvt_slope_;
+  DevBuf<WellDesc> wells_;
+  DevBuf<double> mob_w_, mob_o_, mob_g_, rho_w_, rho_o_, rho_g_, pc_;
+  DevBuf<double> pcog_, div_w_, div_g_, dsw_, dsg_, qw_, qg_, qo_;
+  DevBuf<double> part_, report_;
+  AmgParams amg_prm_;
+  std::unique_ptr<Hierarchy> amg_;
+  Fgmres fgmres_;
+  DevBuf<double> cg_r_, cg_z_, cg_p_, cg_ap_, cg_s_;
+  double dt_ = 0.0;
+};
+
+}  // namespace impes

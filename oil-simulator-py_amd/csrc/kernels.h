@@ -22,6 +22,10 @@ void launch_scale_f32_to_f64(double* y, const double* a, const float* x, long lo
 void launch_scale_f64(double* y, const double* a, const double* x, long long n, hipStream_t s);
 // y += alpha * x  (alpha host scalar)
 void launch_axpy_f64(double* y, double alpha, const double* x, long long n, hipStream_t s);
+// y = x ./ a
+void launch_div_f64(double* y, const double* x, const double* a, long long n, hipStream_t s);
+// y = x + beta * y
+void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipStream_t s);
 // out[0] = sum(x .* y); work >= kRedBlocks doubles
 void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s);
 // out[0] = max |x|

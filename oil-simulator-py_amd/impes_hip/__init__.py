@@ -2,6 +2,9 @@
 from ._lib import (  # noqa: F401
     LIB_PATH,
     AmgParams,
+    FluidStruct,
+    Grid,
+    WellStruct,
     ImpesError,
     SolveStats,
     call,

@@ -53,11 +53,43 @@ class SolveStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class FluidStruct(C.Structure):
+    _fields_ = [
+        ("stone2", C.c_int),
+        ("nw", C.c_double), ("no", C.c_double), ("ng", C.c_double), ("ko_end_w", C.c_double),
+        ("ko_end_g", C.c_double), ("now_", C.c_double), ("nog", C.c_double),
+        ("sw_cr", C.c_double), ("so_r", C.c_double), ("sg_cr", C.c_double),
+        ("mu_o", C.c_double), ("mu_w", C.c_double), ("mu_g", C.c_double),
+        ("rho_o_ref", C.c_double), ("rho_w_ref", C.c_double), ("rho_g_ref", C.c_double),
+        ("c_o", C.c_double), ("c_w", C.c_double), ("c_g", C.c_double), ("c_r", C.c_double), ("cf", C.c_double),
+        ("pc_scale", C.c_double), ("pc_exp", C.c_double), ("pcog_scale", C.c_double), ("pcog_exp", C.c_double),
+        ("g", C.c_double),
+        ("max_ds", C.c_double),
+        ("max_substeps", C.c_int),
+        ("rate_scale", C.c_double),
+    ]
+
+
+class WellStruct(C.Structure):
+    _fields_ = [
+        ("cell", C.c_int), ("producer", C.c_int), ("control", C.c_int), ("gas", C.c_int), ("surface", C.c_int),
+        ("surface_phase", C.c_int), ("has_bhp_min", C.c_int), ("lim_mask", C.c_int),
+        ("value", C.c_double), ("wi", C.c_double), ("bhp_min", C.c_double), ("lim", C.c_double * 4),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _L = C.c_longlong
 _D = C.c_double
 _F = C.c_float
+_PI = C.POINTER(C.c_int)
+_PD = C.POINTER(C.c_double)
 
 # name -> argtypes ; every function returns int unless listed in _RESTYPE
 SIGNATURES = {
@@ -69,6 +101,7 @@ SIGNATURES = {
     "impes_l1jacobi_csr_f64": [_I, _I, _P, _P, _P, _P, _D, _P, _P, _P, _P],
     "impes_l1jacobi_csr_f32": [_I, _I, _P, _P, _P, _P, _F, _P, _P, _P, _P],
     "impes_dot_f64": [_L, _P, _P, _P, _P],
+    "impes_csr_diagonal": [_I, _P, _P, _P, _P, _P],
     "impes_amg_default_params": [C.POINTER(AmgParams)],
     "impes_amg_create": [C.POINTER(AmgParams), C.POINTER(_P)],
     "impes_amg_destroy": [_P],
@@ -89,6 +122,22 @@ SIGNATURES = {
     "impes_fgmres_csr": [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I, C.POINTER(SolveStats), _P],
     "impes_fgmres_st7": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I,
                          C.POINTER(SolveStats), _P],
+    "impes_amg_cycles": [_P, _I, _P, _P, _I, _I, _D, _I, _PD, _PI, _P],
+    "impes_model_create": [C.POINTER(Grid), C.POINTER(FluidStruct), _I, _PD, _I, C.POINTER(WellStruct),
+                           C.POINTER(_P)],
+    "impes_model_destroy": [_P],
+    "impes_model_set_rock": [_P, _P, _P, _P, _P, _P],
+    "impes_model_set_wells": [_P, _I, C.POINTER(WellStruct)],
+    "impes_model_set_amg": [_P, C.POINTER(AmgParams)],
+    "impes_model_report_len": [_P, _PI],
+    "impes_model_assemble": [_P, _D, _P, _P, _P, _P],
+    "impes_model_solve_pressure": [_P, _I, _D, _I, _I, _P, _P, C.POINTER(SolveStats), _P],
+    "impes_model_solve_cg": [_P, _D, _I, _P, _PI, _PI, _P],
+    "impes_model_saturation": [_P, _D, _P, _P, _P, _P, _P, _P, _P, _PD, _P],
+    "impes_model_export": [_P, _I, _P, _P],
+    "impes_model_eval_props": [_P, _L, _P, _P, _P, _P, _P],
+    "impes_model_eval_pvt": [_P, _I, _L, _P, _P, _P],
+    "impes_model_amg_levels": [_P, _PI, _PI, _PI, _I],
 }
 _RESTYPE = {"impes_last_error": C.c_char_p, "impes_amg_default_params": None}
 

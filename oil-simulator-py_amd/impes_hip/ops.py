@@ -13,6 +13,7 @@ from ._lib import AmgParams, ImpesError, SolveStats, call, load, ptr, stream_ptr
 
 __all__ = [
     "spmv_csr", "spmv_st7", "l1jacobi_csr", "dot", "AmgHierarchy", "FgmresSolver", "amg_default_params",
+    "ImpesModel", "csr_diagonal",
 ]
 
 
@@ -55,6 +56,13 @@ def l1jacobi_csr(rowptr, col, val, inv_relax, omega, x_in, b, out=None):
 def dot(x, y):
     out = torch.empty(1, dtype=torch.float64, device=x.device)
     call("impes_dot_f64", x.numel(), ptr(x), ptr(y), ptr(out), stream_ptr())
+    return out
+
+
+def csr_diagonal(rowptr, col, val):
+    n = rowptr.numel() - 1
+    out = torch.empty(n, dtype=torch.float64, device=val.device)
+    call("impes_csr_diagonal", n, ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(out), stream_ptr())
     return out
 
 
@@ -185,6 +193,17 @@ class AmgHierarchy:
         call("impes_amg_precondition", self._h, int(precision), ptr(v), ptr(z), stream_ptr())
         return z
 
+    def cycles(self, b, x=None, *, min_cycles=1, tol=1e-6, max_cycles=10, precision=64):
+        """Stationary V-cycles in physical space (reference ClassicalAMG.solve/apply).
+        Returns (x, relative residual, cycles done)."""
+        use_x0 = x is not None
+        out = x.clone() if use_x0 else torch.empty_like(b)
+        rel = C.c_double()
+        done = C.c_int()
+        call("impes_amg_cycles", self._h, int(precision), ptr(b), ptr(out), 1 if use_x0 else 0, int(min_cycles),
+             float(tol), int(max_cycles), C.byref(rel), C.byref(done), stream_ptr())
+        return out, rel.value, done.value
+
 
 class FgmresSolver:
     """Owner of an ``impes_solver`` handle (FGMRES workspace)."""
@@ -222,3 +241,122 @@ class FgmresSolver:
              ptr(d), ptr(ox), ptr(oy), ptr(oz), ptr(dinv), ptr(b), ptr(x), 1 if use_x0 else 0, float(rtol),
              int(restart), int(maxit), C.byref(st), stream_ptr())
         return st.as_dict()
+
+
+# ---------------------------------------------------------------------- IMPES model
+REPORT_HEAD = 32
+WELL_SLOTS = 12
+EXPORT_FIELDS = {"diag": 0, "ox": 1, "oy": 2, "oz": 3, "rhs": 4, "acc": 5, "extra": 6, "tx": 7, "ty": 8,
+                 "tz": 9, "pv": 10, "mob_t": 11, "dinv": 12}
+PVT_KEYS = {"Bo": 0, "Bw": 1, "Bg": 2, "mu_o": 3, "mu_w": 4, "mu_g": 5, "Rs": 6, "Rv": 7}
+PROP_NAMES = ("kro", "krw", "krg", "pc", "pcog", "ct", "rho_w", "rho_o", "rho_g", "mu_o", "mu_w", "mu_g")
+
+
+class ImpesModel:
+    """Owner of an ``impes_model`` handle: the device-resident IMPES step of one reservoir."""
+
+    def __init__(self, grid: tuple, fluid: "FluidStruct", pvt_table=None, wells=()):
+        from ._lib import FluidStruct, Grid, WellStruct  # noqa: F401
+
+        nx, ny, nz, dx, dy, dz = grid
+        g = Grid(int(nx), int(ny), int(nz), float(dx), float(dy), float(dz))
+        self.n = int(nx) * int(ny) * int(nz)
+        self.dims = (int(nx), int(ny), int(nz))
+        if pvt_table is not None:
+            flat = (C.c_double * pvt_table.size)(*[float(v) for v in pvt_table.ravel()])
+            pn = pvt_table.shape[1]
+        else:
+            flat, pn = None, 0
+        warr, nw = self._wells(wells)
+        h = C.c_void_p()
+        call("impes_model_create", C.byref(g), C.byref(fluid), pn, flat, nw, warr, C.byref(h))
+        self._h = h
+        self.nwells = nw
+        self._report = None
+
+    @staticmethod
+    def _wells(wells):
+        from ._lib import WellStruct
+
+        wells = list(wells)
+        if not wells:
+            return None, 0
+        arr = (WellStruct * len(wells))(*wells)
+        return arr, len(wells)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            try:
+                load().impes_model_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_rock(self, kx, ky, kz, pv):
+        call("impes_model_set_rock", self._h, ptr(kx), ptr(ky), ptr(kz), ptr(pv), stream_ptr())
+
+    def set_wells(self, wells):
+        warr, nw = self._wells(wells)
+        call("impes_model_set_wells", self._h, nw, warr)
+        self.nwells = nw
+        self._report = None
+
+    def set_amg(self, params: AmgParams):
+        call("impes_model_set_amg", self._h, C.byref(params))
+
+    def report_len(self) -> int:
+        v = C.c_int()
+        call("impes_model_report_len", self._h, C.byref(v))
+        return v.value
+
+    def assemble(self, dt, p, sw, sg):
+        call("impes_model_assemble", self._h, float(dt), ptr(p), ptr(sw), ptr(sg), stream_ptr())
+
+    def solve_pressure(self, p_prev, p_new, *, precision=32, rtol=1e-8, restart=30, maxit=300) -> dict:
+        st = SolveStats()
+        call("impes_model_solve_pressure", self._h, int(precision), float(rtol), int(restart), int(maxit),
+             ptr(p_prev), ptr(p_new), C.byref(st), stream_ptr())
+        return st.as_dict()
+
+    def solve_cg(self, p_new, *, tol, maxit) -> tuple[bool, int]:
+        conv, its = C.c_int(), C.c_int()
+        call("impes_model_solve_cg", self._h, float(tol), int(maxit), ptr(p_new), C.byref(conv), C.byref(its),
+             stream_ptr())
+        return bool(conv.value), its.value
+
+    def saturation(self, dt, p_prev, p_new, sw, sg, sw_out, sg_out, so_out=None):
+        """Runs the transport step; returns the raw report (numpy float64 array)."""
+        import numpy as np
+
+        n = self.report_len()
+        if self._report is None or len(self._report) != n:
+            self._report = (C.c_double * n)()
+        call("impes_model_saturation", self._h, float(dt), ptr(p_prev), ptr(p_new), ptr(sw), ptr(sg), ptr(sw_out),
+             ptr(sg_out), ptr(so_out), self._report, stream_ptr())
+        return np.frombuffer(self._report, dtype=np.float64, count=n).copy()
+
+    def export(self, field: str, out=None, device=None):
+        t = out if out is not None else torch.empty(self.n, dtype=torch.float64, device=device or "cuda")
+        call("impes_model_export", self._h, EXPORT_FIELDS[field], ptr(t), stream_ptr())
+        return t
+
+    def eval_props(self, p, sw, sg) -> dict:
+        n = p.numel()
+        out = torch.empty(12 * n, dtype=torch.float64, device=p.device)
+        call("impes_model_eval_props", self._h, n, ptr(p), ptr(sw), ptr(sg), ptr(out), stream_ptr())
+        return {k: out[i * n:(i + 1) * n] for i, k in enumerate(PROP_NAMES)}
+
+    def eval_pvt(self, key: str, p):
+        out = torch.empty_like(p)
+        call("impes_model_eval_pvt", self._h, PVT_KEYS[key], p.numel(), ptr(p), ptr(out), stream_ptr())
+        return out
+
+    def amg_levels(self, max_levels=64):
+        nl = C.c_int()
+        sizes = (C.c_int * max_levels)()
+        nnz = (C.c_int * max_levels)()
+        call("impes_model_amg_levels", self._h, C.byref(nl), sizes, nnz, max_levels)
+        return [(sizes[i], nnz[i]) for i in range(min(nl.value, max_levels))]

@@ -265,14 +265,20 @@ class Hierarchy:
         self._coarse_factor()
 
     def _coarse_factor(self):
+        """classical_amg.py:1045-1069: dense solve with DOF 0 pinned. The pin is
+        applied only when the hierarchy has more than one level: alone it would
+        freeze DOF 0 of a nonsingular system (a single-level hierarchy is solved
+        exactly instead)."""
         L = self.levels[-1]
         n = L["A"].shape[0]
         L["dense"] = n <= self.dense_max
+        L["pin"] = len(self.levels) > 1
         if L["dense"]:
             Ad = L["A"].toarray()
-            Ad[0, :] = 0.0
-            Ad[:, 0] = 0.0
-            Ad[0, 0] = 1.0
+            if L["pin"]:
+                Ad[0, :] = 0.0
+                Ad[:, 0] = 0.0
+                Ad[0, 0] = 1.0
             L["Ainv"] = np.linalg.inv(Ad)
 
     def update(self, A0: sp.csr_matrix):
@@ -296,12 +302,11 @@ class Hierarchy:
         return x
 
     def _coarse(self, L, b):
-        if L["dense"]:
-            bl = b.copy()
-            bl[0] = 0.0
-            return L["Ainv"] @ bl
         bl = b.copy()
-        bl[0] = 0.0
+        if L["pin"]:
+            bl[0] = 0.0
+        if L["dense"]:
+            return L["Ainv"] @ bl
         return self._smooth(L, np.zeros_like(b), bl, self.coarse_sweeps)
 
     def vcycle(self, b, x=None, lvl=0):
