@@ -14,6 +14,24 @@
 
 namespace impes {
 
+// Scaling of the fine operator when its lower-median |a_ii| is below equil_threshold.
+// Symmetric: D^-1/2 A D^-1/2 with D = |diag A| (classical_amg.py:699-760), recomputed on
+// every update. Scalar: A / median|a_ii|, frozen at setup; it keeps the constant vector
+// as the near-nullspace that row-normalised interpolation reproduces.
+constexpr int kEquilSymmetric = 0;
+constexpr int kEquilScalar = 1;
+// Coarsest-level DOF-0 pin: never; whenever the hierarchy has more than one level
+// (classical_amg.py:1045-1069); or only when the coarsest operator is singular, i.e. every
+// row has |row sum| <= 1e-8 |a_ii| (decided on the device at every setup/update).
+// C/F split: one pass of strict local maxima plus lift (classical_amg.py:85-146), or
+// rounds of that selection among the still-undecided points until all are decided
+// (an independent-set split; coarsens far more on the dense operators of deep levels).
+constexpr int kCoarsenSinglePass = 0;
+constexpr int kCoarsenIterated = 1;
+constexpr int kPinNever = 0;
+constexpr int kPinReference = 1;
+constexpr int kPinAuto = 2;
+
 struct AmgParams {
   double theta = 0.25;
   int max_levels = 10;
@@ -25,6 +43,12 @@ struct AmgParams {
   int dense_max = 500;            // largest coarsest level solved densely
   int coarse_sweeps = 50;         // Jacobi sweeps on a coarsest level above dense_max
   int anchor = -1;                // forced C point on every level (reference anchor_idx), -1 = none
+  // Additions that change the preconditioner (not the converged answer).
+  int equil_mode = kEquilScalar;  // scaling applied when the median test fires
+  int coarse_pin = kPinAuto;      // DOF-0 pin of the coarsest level
+  double trunc = 0.2;             // drop F-row weights below trunc * row max (0 = keep all)
+  int pmax = 4;                   // keep at most pmax weights per F row (0 = unlimited)
+  int coarsening = kCoarsenIterated;
 };
 
 struct LevelInfo {
@@ -33,7 +57,9 @@ struct LevelInfo {
 
 struct Level {
   int n = 0, nnz = 0, group = 1;
-  bool coarsest = false, dense = false, pin = true;
+  bool coarsest = false, dense = false;
+  int pin_mode = kPinNever;  // effective pin mode of the coarsest level
+  DevBuf<int> pin;           // device flag: 1 when DOF 0 is pinned
   DevBuf<int> rp, ci;
   DevBuf<double> v;
   DevBuf<float> v32;
@@ -92,6 +118,7 @@ class Hierarchy {
   int num_levels() const { return (int)levels_.size(); }
   LevelInfo info(int l) const;
   bool equilibrated() const { return equil_; }
+  bool coarse_pinned(hipStream_t s) const;  // reads the device pin flag
   const double* dhalf_inv() const { return dh_.p; }  // D^-1/2 or ones
   const double* diag0() const { return diag0_.p; }   // unscaled fine diagonal
   const Level& level(int l) const { return levels_[l]; }
@@ -101,7 +128,7 @@ class Hierarchy {
 
  private:
   void setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s);
-  void equilibrate(const double* v_unscaled, hipStream_t s);
+  void equilibrate(const double* v_unscaled, bool decide, hipStream_t s);
   void build_levels(hipStream_t s);
   void finish_level_values(int l, hipStream_t s);
   void build_coarsest(int l, hipStream_t s);
@@ -118,6 +145,7 @@ class Hierarchy {
   AmgParams prm_;
   std::vector<Level> levels_;
   bool equil_ = false;
+  double scalar_ = 1.0;  // frozen scale of kEquilScalar
   DevBuf<double> v0_unscaled_;  // fine values as given
   DevBuf<double> diag0_, dh_, sin_, sout_;
   // stencil form of level 0

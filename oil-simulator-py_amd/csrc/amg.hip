@@ -1,7 +1,12 @@
 // Host orchestration of the RS-AMG hierarchy: setup, numeric update and V-cycle.
 #include "amg.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
 
 #include "amg_kernels.h"
 #include "prims.h"
@@ -93,8 +98,6 @@ __global__ void k_st7_scale(int nx, int ny, int nz, const double* __restrict__ d
   }
 }
 
-__global__ void k_set_pinned_zero(double* __restrict__ b) { b[0] = 0.0; }
-__global__ void k_set_pinned_zero32(float* __restrict__ b) { b[0] = 0.f; }
 __global__ void k_set_cf(signed char* cf, int idx) { cf[idx] = kC; }
 __global__ void k_neg_flags(int m, const int* __restrict__ tp, int* __restrict__ out) {
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < m; e += (long long)gridDim.x * blockDim.x)
@@ -127,8 +130,8 @@ struct Sel<double> {
     launch_spmv_add_f64(A, x, y, g, s);
   }
   static void copy(double* y, const double* x, long long n, hipStream_t s) { launch_copy_f64(y, x, n, s); }
-  static void pin0(double* b, hipStream_t s) { k_set_pinned_zero<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
-  static void dense(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s) {
+  static void pin0(const int* pin, double* b, hipStream_t s) { launch_pin_rhs_f64(pin, b, s); }
+  static void dense(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s) {
     launch_dense_solve_f64(n, pin, aug, b, x, s);
   }
 };
@@ -155,8 +158,8 @@ struct Sel<float> {
     launch_spmv_add_f32(A, x, y, g, s);
   }
   static void copy(float* y, const float* x, long long n, hipStream_t s) { launch_copy_f32(y, x, n, s); }
-  static void pin0(float* b, hipStream_t s) { k_set_pinned_zero32<<<1, 1, 0, s>>>(b); IMPES_LAUNCH_CHECK(); }
-  static void dense(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s) {
+  static void pin0(const int* pin, float* b, hipStream_t s) { launch_pin_rhs_f32(pin, b, s); }
+  static void dense(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s) {
     launch_dense_solve_f32(n, pin, aug, b, x, s);
   }
 };
@@ -201,7 +204,7 @@ void Hierarchy::setup_impl(int n, int nnz, const int* rp, const int* ci, const d
   d2d(L0.ci.p, ci, nnz, s);
   v0_unscaled_.alloc(nnz);
   d2d(v0_unscaled_.p, v, nnz, s);
-  equilibrate(v0_unscaled_.p, s);
+  equilibrate(v0_unscaled_.p, true, s);
   build_levels(s);
   refresh_f32(s);
   if (stencil_) refresh_stencil(s);
@@ -232,7 +235,7 @@ void Hierarchy::setup_stencil(int nx, int ny, int nz, const double* d, const dou
   setup_impl((int)n, nnz, rp.p, ci.p, v.p, s);
 }
 
-void Hierarchy::equilibrate(const double* vu, hipStream_t s) {
+void Hierarchy::equilibrate(const double* vu, bool decide, hipStream_t s) {
   Level& L0 = levels_[0];
   const int n = L0.n;
   diag0_.alloc(n);
@@ -240,26 +243,46 @@ void Hierarchy::equilibrate(const double* vu, hipStream_t s) {
   sin_.alloc(n);
   sout_.alloc(n);
   launch_diag_relax(n, L0.rp.p, L0.ci.p, vu, diag0_.p, nullptr, nullptr, s);
-  DevBuf<int> flags;
-  flags.alloc(n);
-  launch_count_small_diag(n, diag0_.p, prm_.equil_threshold, flags.p, s);
-  // lower median (torch.median semantics) below the threshold  <=>  at least
-  // (n-1)/2 + 1 entries below it
-  const long long small = sum_i32(flags.p, n, s);
-  equil_ = small >= (long long)((n - 1) / 2 + 1);
-  if (equil_) {
-    launch_dhalf_inv(n, diag0_.p, dh_.p, s);
+  if (prm_.equil_mode == kEquilScalar) {
+    if (decide) {
+      // lower median of |a_ii| (torch.median semantics), chosen once per setup
+      std::vector<double> d((size_t)n);
+      IMPES_HIP_CHECK(hipMemcpyAsync(d.data(), diag0_.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      for (auto& x : d) x = std::max(std::fabs(x), 1e-30);
+      auto mid = d.begin() + (n - 1) / 2;
+      std::nth_element(d.begin(), mid, d.end());
+      equil_ = *mid < prm_.equil_threshold;
+      scalar_ = equil_ ? 1.0 / std::sqrt(*mid) : 1.0;
+    }
+    launch_fill_f64(dh_.p, scalar_, n, s);
     launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
   } else {
-    launch_fill_f64(dh_.p, 1.0, n, s);
-    d2d(L0.v.p, vu, L0.nnz, s);
+    DevBuf<int> flags;
+    flags.alloc(n);
+    launch_count_small_diag(n, diag0_.p, prm_.equil_threshold, flags.p, s);
+    // lower median (torch.median semantics) below the threshold  <=>  at least
+    // (n-1)/2 + 1 entries below it
+    const long long small = sum_i32(flags.p, n, s);
+    equil_ = small >= (long long)((n - 1) / 2 + 1);
+    if (equil_) {
+      launch_dhalf_inv(n, diag0_.p, dh_.p, s);
+      launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
+    } else {
+      launch_fill_f64(dh_.p, 1.0, n, s);
+      d2d(L0.v.p, vu, L0.nnz, s);
+    }
+    IMPES_HIP_CHECK(hipStreamSynchronize(s));  // flags freed on return
   }
   launch_scale_f64(sin_.p, dh_.p, diag0_.p, n, s);
   launch_copy_f64(sout_.p, dh_.p, n, s);
-  IMPES_HIP_CHECK(hipStreamSynchronize(s));  // flags freed on return
 }
 
 void Hierarchy::build_levels(hipStream_t s) {
+  // IMPES_AMG_TRACE=1 prints one line per level built (size, pattern, seconds) to stderr
+  const char* tr = std::getenv("IMPES_AMG_TRACE");
+  const bool trace = tr && tr[0] == '1';
+  double tl = wall();
   for (int lvl = 0; lvl < prm_.max_levels; ++lvl) {
     const int n = levels_[lvl].n;
     if (n <= prm_.coarsest_size) {
@@ -290,7 +313,13 @@ void Hierarchy::build_levels(hipStream_t s) {
     deg.alloc(n);
     is_max.alloc(n);
     cf0.alloc(n);
-    launch_cf_split(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
+    if (prm_.coarsening == kCoarsenIterated) {
+      DevBuf<int> undecided;
+      undecided.alloc(n);
+      launch_cf_split_iterated(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, undecided.p, cf0.p, s);
+    } else {
+      launch_cf_split(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
+    }
     if (prm_.anchor >= 0 && prm_.anchor < n) {
       k_set_cf<<<1, 1, 0, s>>>(cf0.p, prm_.anchor);
       IMPES_LAUNCH_CHECK();
@@ -325,7 +354,8 @@ void Hierarchy::build_levels(hipStream_t s) {
     oc.alloc(ncand);
     ov.alloc(ncand);
     pcnt.alloc(n);
-    launch_interp_reduce(n, A.rp.p, A.ci.p, A.cf.p, f2c.p, cand_ptr.p, keys.p, cvals.p, oc.p, ov.p, pcnt.p, s);
+    launch_interp_reduce(n, A.rp.p, A.ci.p, A.cf.p, f2c.p, cand_ptr.p, keys.p, cvals.p, prm_.trunc, prm_.pmax, oc.p,
+                         ov.p, pcnt.p, s);
     A.prp.alloc(n + 1);
     scan_i32_to_i32(pcnt.p, A.prp.p, n, s);
     A.pnnz = read_scalar(A.prp.p + n, s);
@@ -391,6 +421,12 @@ void Hierarchy::build_levels(hipStream_t s) {
     A.pgroup = csr_group_for(n, A.pnnz);
     A.rgroup = csr_group_for(nc, A.pnnz);
     IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    if (trace) {
+      const double now = wall();
+      std::fprintf(stderr, "[amg] level %d n=%d nnz=%d -> nc=%d p_nnz=%d ac_nnz=%d  %.3f s\n", lvl, n, nnz, nc,
+                   A.pnnz, nxt.nnz, now - tl);
+      tl = now;
+    }
     levels_.push_back(std::move(nxt));
   }
   if (!levels_.back().coarsest) levels_.back().coarsest = true;
@@ -409,10 +445,13 @@ void Hierarchy::finish_level_values(int l, hipStream_t s) {
 void Hierarchy::build_coarsest(int l, hipStream_t s) {
   Level& L = levels_[l];
   L.dense = L.n <= prm_.dense_max;
-  L.pin = levels_.size() > 1;
+  // a single-level hierarchy is solved without the pin: alone it would freeze DOF 0
+  L.pin_mode = levels_.size() > 1 ? prm_.coarse_pin : kPinNever;
+  L.pin.alloc(1);
+  launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
   if (!L.dense) return;
   L.aug.alloc(2LL * L.n * L.n);
-  launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin ? 1 : 0, L.aug.p, s);
+  launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
   launch_gauss_jordan_inverse(L.n, L.aug.p, s);
 }
 
@@ -450,7 +489,7 @@ void Hierarchy::alloc_vectors() {
 void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "amg update before setup");
   if (v != v0_unscaled_.p) d2d(v0_unscaled_.p, v, levels_[0].nnz, s);
-  equilibrate(v0_unscaled_.p, s);
+  equilibrate(v0_unscaled_.p, false, s);
   for (int l = 0; l + 1 < (int)levels_.size(); ++l) {
     Level& A = levels_[l];
     if (A.coarsest) break;
@@ -462,9 +501,12 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
     Level& L = levels_[l];
     launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
     launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
-    if (L.coarsest && L.dense) {
-      launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin ? 1 : 0, L.aug.p, s);
-      launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+    if (L.coarsest) {
+      launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
+      if (L.dense) {
+        launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
+        launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+      }
     }
   }
   if (stencil_) refresh_stencil(s);
@@ -477,6 +519,14 @@ void Hierarchy::update_stencil(const double* d, const double* ox, const double* 
   Level& L0 = levels_[0];
   launch_st7_to_csr_values(nx_, ny_, nz_, d, ox, oy, oz, L0.rp.p, v0_unscaled_.p, s);
   update_csr_values(v0_unscaled_.p, s);
+}
+
+bool Hierarchy::coarse_pinned(hipStream_t s) const {
+  IMPES_REQUIRE(!levels_.empty(), "amg hierarchy not set up");
+  int h = 0;
+  IMPES_HIP_CHECK(hipMemcpyAsync(&h, levels_.back().pin.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  return h != 0;
 }
 
 LevelInfo Hierarchy::info(int l) const {
@@ -514,13 +564,13 @@ template <class T>
 void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
   Level& L = levels_[l];
   if (L.dense) {
-    Sel<T>::dense(L.n, L.pin ? 1 : 0, L.aug.p, b, x, s);
+    Sel<T>::dense(L.n, L.pin.p, L.aug.p, b, x, s);
     return;
   }
   // reference: 50 L1-Jacobi sweeps from zero on the unpinned matrix with b_0 = 0
   T* bl = Sel<T>::r(L);
   Sel<T>::copy(bl, b, L.n, s);
-  if (L.pin) Sel<T>::pin0(bl, s);
+  Sel<T>::pin0(L.pin.p, bl, s);
   T* cur = x;
   T* other = Sel<T>::t(L);
   bool zero = true;

@@ -27,6 +27,11 @@ void launch_transpose_pos(int n, const int* rp, const int* ci, int* tpos, hipStr
 // C/F split: symmetrised strong graph, strict local maxima of (deg, hash, index), lift of F points with no C neighbour
 void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos, int* deg,
                      signed char* is_max, signed char* cf, hipStream_t s);
+// iterated independent-set split on the symmetrised strong graph with the same
+// (deg, hash, index) order: rounds of local-max selection among undecided points,
+// whose undecided neighbours become F, until every point is decided (no lift needed)
+void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos,
+                              int* deg, signed char* newc, int* undecided, signed char* cf, hipStream_t s);
 // promote F points with no strong C neighbour
 void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
                     signed char* cf_out, int* is_c, hipStream_t s);
@@ -39,9 +44,12 @@ void launch_interp_count(int n, const int* rp, const int* ci, const unsigned cha
 void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
                           const signed char* cf, const double* sum_strong, const int* cnt_c, const int* f2c,
                           const long long* cand_ptr, unsigned long long* keys, double* vals, hipStream_t s);
+// final F-row weights; trunc > 0 / pmax > 0 truncate each F row afterwards: weights below
+// trunc * row max are dropped, then all but the pmax largest (ties: lower coarse column
+// first), and the kept weights are rescaled to the row's previous sum
 void launch_interp_reduce(int n, const int* rp, const int* ci, const signed char* cf, const int* f2c,
                           const long long* cand_ptr, const unsigned long long* keys, const double* vals,
-                          int* out_cols, double* out_vals, int* pcnt, hipStream_t s);
+                          double trunc, int pmax, int* out_cols, double* out_vals, int* pcnt, hipStream_t s);
 void launch_interp_gather(int n, const long long* cand_ptr, const int* cols_tmp, const double* vals_tmp,
                           const int* prp, int* pci, double* pv, hipStream_t s);
 
@@ -68,11 +76,19 @@ void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mr
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
                            const double* mv, double* cv, hipStream_t s);
 
-// coarsest level (coarse.hip): dense inverse of the (optionally DOF-0-pinned) matrix
-void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, int pin, double* dense, hipStream_t s);
+// coarsest level (coarse.hip). pin is a device flag written by launch_pin_decide and
+// read by the kernels below, so the decision never leaves the device.
+// pin = (mode == reference) || (mode == auto && every row has |row sum| <= 1e-8 |a_ii|)
+void launch_pin_decide(int n, const int* rp, const int* ci, const double* v, int mode, int* pin, hipStream_t s);
+// dense inverse of the (optionally DOF-0-pinned) matrix
+void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, const int* pin, double* dense,
+                         hipStream_t s);
 void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s);  // aug is n x 2n row-major, inverse in right half
 // x = Ainv * b (with pin: b[0] treated as 0); FP64 and FP32 vector variants
-void launch_dense_solve_f64(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s);
-void launch_dense_solve_f32(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s);
+void launch_dense_solve_f64(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s);
+void launch_dense_solve_f32(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s);
+// b[0] = 0 when *pin (Jacobi coarsest path)
+void launch_pin_rhs_f64(const int* pin, double* b, hipStream_t s);
+void launch_pin_rhs_f32(const int* pin, float* b, hipStream_t s);
 
 }  // namespace impes

@@ -10,6 +10,7 @@
 // The unseeded torch.rand tie-break of the C/F split (:125) is replaced by a
 // deterministic integer hash of the row index, so the hierarchy is reproducible.
 #include "amg_kernels.h"
+#include "prims.h"
 
 namespace impes {
 
@@ -149,6 +150,52 @@ __global__ void k_cf_lift(int n, const int* __restrict__ rp, const int* __restri
   }
 }
 
+// Iterated independent-set split (not in the reference): every round selects the
+// undecided points that rank above all their undecided graph neighbours as C, and
+// turns the undecided neighbours of a new C point into F, until none is undecided.
+constexpr signed char kUndecided = 2;
+
+__global__ void k_mis_init(int n, signed char* __restrict__ cf) {
+  ROW_LOOP(i, n) cf[i] = kUndecided;
+}
+
+__global__ void k_mis_select(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                             const int* __restrict__ deg, const signed char* __restrict__ cf,
+                             signed char* __restrict__ newc) {
+  ROW_LOOP(i, n) {
+    bool mx = cf[i] == kUndecided;
+    const int di = deg[i];
+    for (int e = rp[i]; e < rp[i + 1] && mx; ++e) {
+      const int j = ci[e];
+      if (j != (int)i && cf[j] == kUndecided && in_graph(strong, tpos, e) && ranks_above(deg[j], j, di, (int)i))
+        mx = false;
+    }
+    newc[i] = mx ? 1 : 0;
+  }
+}
+
+__global__ void k_mis_cover(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                            const signed char* __restrict__ newc, signed char* __restrict__ cf,
+                            int* __restrict__ undecided) {
+  ROW_LOOP(i, n) {
+    int u = 0;
+    if (newc[i]) {
+      cf[i] = kC;
+    } else if (cf[i] == kUndecided) {
+      bool cov = false;
+      for (int e = rp[i]; e < rp[i + 1] && !cov; ++e) {
+        const int j = ci[e];
+        if (j != (int)i && newc[j] && in_graph(strong, tpos, e)) cov = true;
+      }
+      if (cov) cf[i] = kF;
+      else u = 1;
+    }
+    undecided[i] = u;
+  }
+}
+
 __global__ void k_orphans(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                           const unsigned char* __restrict__ strong, const signed char* __restrict__ cf_in,
                           signed char* __restrict__ cf_out, int* __restrict__ is_c) {
@@ -274,8 +321,8 @@ __global__ void k_interp_expand(int n, const int* __restrict__ rp, const int* __
 __global__ void k_interp_reduce(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                 const signed char* __restrict__ cf, const int* __restrict__ f2c,
                                 const long long* __restrict__ cand_ptr, const unsigned long long* __restrict__ keys,
-                                const double* __restrict__ vals, int* __restrict__ oc, double* __restrict__ ov,
-                                int* __restrict__ pcnt) {
+                                const double* __restrict__ vals, double trunc, int pmax, int* __restrict__ oc,
+                                double* __restrict__ ov, int* __restrict__ pcnt) {
   ROW_LOOP(i, n) {
     const long long b = cand_ptr[i], e_end = cand_ptr[i + 1];
     if (cf[i] == kC) {
@@ -335,6 +382,51 @@ __global__ void k_interp_reduce(int n, const int* __restrict__ rp, const int* __
         ov[b + w] = val;
         ++w;
       }
+    }
+    if (w > 1 && (trunc > 0.0 || pmax > 0)) {
+      // truncation (not in the reference). Weights within tie = 1e-10 * row max of each
+      // other count as equal (ties go to the lower coarse column), so that rounding
+      // differences of a few ulps cannot change which weights survive.
+      double tot = 0.0, mx = 0.0;
+      for (int q = 0; q < w; ++q) {
+        tot += ov[b + q];
+        mx = fmax(mx, ov[b + q]);
+      }
+      const double tie = 1e-10 * mx;
+      const double cut = trunc * mx;
+      int k = 0;
+      for (int q = 0; q < w; ++q)
+        if (ov[b + q] + tie >= cut) {
+          oc[b + k] = oc[b + q];
+          ov[b + k] = ov[b + q];
+          ++k;
+        }
+      if (pmax > 0 && k > pmax) {
+        // rank of q = entries above it (by more than tie) + tied entries to its left;
+        // ranks only read values, so dropped entries are flagged in the column array
+        for (int q = 0; q < k; ++q) {
+          const double vq = ov[b + q];
+          int rank = 0;
+          for (int r = 0; r < k; ++r) {
+            const double vr = ov[b + r];
+            rank += (vr > vq + tie || (fabs(vr - vq) <= tie && r < q)) ? 1 : 0;
+          }
+          if (rank >= pmax) oc[b + q] = -1 - oc[b + q];
+        }
+        int m = 0;
+        for (int q = 0; q < k; ++q)
+          if (oc[b + q] >= 0) {
+            oc[b + m] = oc[b + q];
+            ov[b + m] = ov[b + q];
+            ++m;
+          }
+        k = m;
+      }
+      double ks = 0.0;
+      for (int q = 0; q < k; ++q) ks += ov[b + q];
+      const double f = tot / ks;
+      for (int q = 0; q < k; ++q) ov[b + q] = ov[b + q] * f;
+      w = k;
     }
     pcnt[i] = w;
   }
@@ -503,6 +595,18 @@ void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* s
   launch_rows(k_cf_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, is_max);
   launch_rows(k_cf_lift, n, s, n, rp, ci, strong, tpos, (const signed char*)is_max, cf);
 }
+void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos,
+                              int* deg, signed char* newc, int* undecided, signed char* cf, hipStream_t s) {
+  launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
+  launch_rows(k_mis_init, n, s, n, cf);
+  // the undecided point of highest rank is selected in every round, so at most n rounds
+  for (long long round = 0;; ++round) {
+    IMPES_REQUIRE(round <= n, "amg setup: independent-set split does not terminate");
+    launch_rows(k_mis_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, (const signed char*)cf, newc);
+    launch_rows(k_mis_cover, n, s, n, rp, ci, strong, tpos, (const signed char*)newc, cf, undecided);
+    if (sum_i32(undecided, n, s) == 0) break;
+  }
+}
 void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
                     signed char* cf_out, int* is_c, hipStream_t s) {
   launch_rows(k_orphans, n, s, n, rp, ci, strong, cf_in, cf_out, is_c);
@@ -522,8 +626,8 @@ void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, 
 }
 void launch_interp_reduce(int n, const int* rp, const int* ci, const signed char* cf, const int* f2c,
                           const long long* cand_ptr, const unsigned long long* keys, const double* vals,
-                          int* out_cols, double* out_vals, int* pcnt, hipStream_t s) {
-  launch_rows(k_interp_reduce, n, s, n, rp, ci, cf, f2c, cand_ptr, keys, vals, out_cols, out_vals, pcnt);
+                          double trunc, int pmax, int* out_cols, double* out_vals, int* pcnt, hipStream_t s) {
+  launch_rows(k_interp_reduce, n, s, n, rp, ci, cf, f2c, cand_ptr, keys, vals, trunc, pmax, out_cols, out_vals, pcnt);
 }
 void launch_interp_gather(int n, const long long* cand_ptr, const int* cols_tmp, const double* vals_tmp,
                           const int* prp, int* pci, double* pv, hipStream_t s) {

@@ -29,6 +29,37 @@ struct impes_solver {
 
 static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// impes_amg_params -> AmgParams, with range checks shared by every entry point
+static AmgParams params_from_c(const impes_amg_params* c) {
+  AmgParams p;
+  if (c) {
+    p.theta = c->theta;
+    p.max_levels = c->max_levels;
+    p.coarsest_size = c->coarsest_size;
+    p.pre_smooth = c->pre_smooth;
+    p.post_smooth = c->post_smooth;
+    p.omega = c->omega;
+    p.equil_threshold = c->equil_threshold;
+    p.dense_max = c->dense_max;
+    p.coarse_sweeps = c->coarse_sweeps;
+    p.anchor = c->anchor;
+    p.equil_mode = c->equil_mode;
+    p.coarse_pin = c->coarse_pin;
+    p.trunc = c->trunc;
+    p.pmax = c->pmax;
+    p.coarsening = c->coarsening;
+  }
+  IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
+  IMPES_REQUIRE(p.pre_smooth >= 0 && p.post_smooth >= 0, "negative sweep count");
+  IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= 512, "dense_max must be in [1, 512]");
+  IMPES_REQUIRE(p.equil_mode == kEquilSymmetric || p.equil_mode == kEquilScalar, "equil_mode must be 0 or 1");
+  IMPES_REQUIRE(p.coarse_pin >= kPinNever && p.coarse_pin <= kPinAuto, "coarse_pin must be 0, 1 or 2");
+  IMPES_REQUIRE(p.trunc >= 0.0 && p.trunc < 1.0, "trunc must be in [0, 1)");
+  IMPES_REQUIRE(p.pmax >= 0, "pmax must be >= 0");
+  IMPES_REQUIRE(p.coarsening == kCoarsenSinglePass || p.coarsening == kCoarsenIterated, "coarsening must be 0 or 1");
+  return p;
+}
+
 static void check_csr(int n_rows, int nnz, const int* rowptr, const int* col, const void* val) {
   IMPES_REQUIRE(n_rows >= 0 && nnz >= 0, "negative size");
   IMPES_REQUIRE(rowptr && (nnz == 0 || (col && val)), "null CSR pointer");
@@ -123,27 +154,17 @@ void impes_amg_default_params(impes_amg_params* p) {
   p->dense_max = d.dense_max;
   p->coarse_sweeps = d.coarse_sweeps;
   p->anchor = d.anchor;
+  p->equil_mode = d.equil_mode;
+  p->coarse_pin = d.coarse_pin;
+  p->trunc = d.trunc;
+  p->pmax = d.pmax;
+  p->coarsening = d.coarsening;
 }
 
 int impes_amg_create(const impes_amg_params* params, impes_amg** out) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(out, "null out");
-    AmgParams p;
-    if (params) {
-      p.theta = params->theta;
-      p.max_levels = params->max_levels;
-      p.coarsest_size = params->coarsest_size;
-      p.pre_smooth = params->pre_smooth;
-      p.post_smooth = params->post_smooth;
-      p.omega = params->omega;
-      p.equil_threshold = params->equil_threshold;
-      p.dense_max = params->dense_max;
-      p.coarse_sweeps = params->coarse_sweeps;
-      p.anchor = params->anchor;
-    }
-    IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
-    IMPES_REQUIRE(p.pre_smooth >= 0 && p.post_smooth >= 0, "negative sweep count");
-    IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= 512, "dense_max must be in [1, 512]");
+    const AmgParams p = params_from_c(params);
     *out = new impes_amg(p);
   });
 }
@@ -432,19 +453,7 @@ int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_ho
 int impes_model_set_amg(impes_model* m, const impes_amg_params* params) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(m && params, "null pointer");
-    AmgParams p;
-    p.theta = params->theta;
-    p.max_levels = params->max_levels;
-    p.coarsest_size = params->coarsest_size;
-    p.pre_smooth = params->pre_smooth;
-    p.post_smooth = params->post_smooth;
-    p.omega = params->omega;
-    p.equil_threshold = params->equil_threshold;
-    p.dense_max = params->dense_max;
-    p.coarse_sweeps = params->coarse_sweeps;
-    p.anchor = params->anchor;
-    IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
-    IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= 512, "dense_max must be in [1, 512]");
+    const AmgParams p = params_from_c(params);
     m->m.set_amg_params(p);
   });
 }

@@ -1,10 +1,11 @@
 // Coarsest-level direct solve of the V-cycle (classical_amg.py:1045-1069):
-// the level matrix is made dense, DOF 0 is pinned (row/column 0 zeroed, a_00 = 1,
-// b_0 = 0) when the hierarchy has more than one level, and the system is solved
-// exactly. A single-level hierarchy is solved without the pin: the pin exists for
-// singular (pure Neumann) matrices, and on its own it would freeze DOF 0. The inverse is formed once per
-// hierarchy update by Gauss-Jordan elimination with partial pivoting in FP64, so
-// each V-cycle visit is a single dense matrix-vector product.
+// the level matrix is made dense, DOF 0 is optionally pinned (row/column 0 zeroed,
+// a_00 = 1, b_0 = 0), and the system is solved exactly. The pin exists for singular
+// (pure Neumann) matrices; on a nonsingular but nearly singular operator it removes
+// the near-nullspace correction, so the pin decision is a device flag computed from
+// the level's row sums (launch_pin_decide). The inverse is formed once per hierarchy
+// update by Gauss-Jordan elimination with partial pivoting in FP64, so each V-cycle
+// visit is a single dense matrix-vector product.
 #include "amg_kernels.h"
 
 namespace impes {
@@ -21,9 +22,34 @@ __global__ void k_dense_identity(int n, double* __restrict__ aug) {
   }
 }
 
+// One workgroup: pin = mode==1 || (mode==2 && no row has |row sum| > 1e-8 |a_ii|).
+__global__ __launch_bounds__(256) void k_pin_decide(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const double* __restrict__ v, int mode, int* __restrict__ pin) {
+  int regular = 0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double rs = 0.0, d = 0.0;
+    for (int e = rp[r]; e < rp[r + 1]; ++e) {
+      rs += v[e];
+      if (ci[e] == r) d += v[e];
+    }
+    if (fabs(rs) > 1e-8 * fabs(d)) regular = 1;
+  }
+  regular = __syncthreads_or(regular);
+  if (threadIdx.x == 0) *pin = (mode == 1) ? 1 : (mode == 2 ? !regular : 0);
+}
+
+__global__ void k_pin_rhs(const int* __restrict__ pin, double* __restrict__ b) {
+  if (*pin) b[0] = 0.0;
+}
+__global__ void k_pin_rhs32(const int* __restrict__ pin, float* __restrict__ b) {
+  if (*pin) b[0] = 0.f;
+}
+
 __global__ void k_dense_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                const double* __restrict__ v, int pin, double* __restrict__ aug) {
+                                const double* __restrict__ v, const int* __restrict__ pin_flag,
+                                double* __restrict__ aug) {
   const long long w = 2LL * n;
+  const int pin = *pin_flag;
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
     for (int e = rp[r]; e < rp[r + 1]; ++e) {
       const int c = ci[e];
@@ -93,9 +119,11 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_dense_solve(int n, int pin, const double* __restrict__ aug,
-                                                     const T* __restrict__ b, T* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_dense_solve(int n, const int* __restrict__ pin_flag,
+                                                     const double* __restrict__ aug, const T* __restrict__ b,
+                                                     T* __restrict__ x) {
   const long long w = 2LL * n;
+  const int pin = *pin_flag;
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int nwaves = (gridDim.x * blockDim.x) / kWave;
@@ -109,7 +137,23 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, int pin, const doubl
 
 }  // namespace
 
-void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, int pin, double* aug, hipStream_t s) {
+void launch_pin_decide(int n, const int* rp, const int* ci, const double* v, int mode, int* pin, hipStream_t s) {
+  k_pin_decide<<<1, 256, 0, s>>>(n, rp, ci, v, mode, pin);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_pin_rhs_f64(const int* pin, double* b, hipStream_t s) {
+  k_pin_rhs<<<1, 1, 0, s>>>(pin, b);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_pin_rhs_f32(const int* pin, float* b, hipStream_t s) {
+  k_pin_rhs32<<<1, 1, 0, s>>>(pin, b);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, const int* pin, double* aug,
+                         hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
   const long long tot = 2LL * n * n;
   k_dense_identity<<<grid_for(tot, 256), 256, 0, s>>>(n, aug);
@@ -124,12 +168,12 @@ void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s) {
   IMPES_LAUNCH_CHECK();
 }
 
-void launch_dense_solve_f64(int n, int pin, const double* aug, const double* b, double* x, hipStream_t s) {
+void launch_dense_solve_f64(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s) {
   k_dense_solve<double><<<div_up(n, 4), 256, 0, s>>>(n, pin, aug, b, x);
   IMPES_LAUNCH_CHECK();
 }
 
-void launch_dense_solve_f32(int n, int pin, const double* aug, const float* b, float* x, hipStream_t s) {
+void launch_dense_solve_f32(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s) {
   k_dense_solve<float><<<div_up(n, 4), 256, 0, s>>>(n, pin, aug, b, x);
   IMPES_LAUNCH_CHECK();
 }

@@ -35,6 +35,11 @@ class AmgParams(C.Structure):
         ("dense_max", C.c_int),
         ("coarse_sweeps", C.c_int),
         ("anchor", C.c_int),
+        ("equil_mode", C.c_int),
+        ("coarse_pin", C.c_int),
+        ("trunc", C.c_double),
+        ("pmax", C.c_int),
+        ("coarsening", C.c_int),
     ]
 
 

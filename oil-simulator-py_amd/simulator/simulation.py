@@ -64,6 +64,10 @@ class Simulator:
         p.theta = float(self._amg_cfg.get("theta", 0.25))
         p.max_levels = int(self._amg_cfg.get("max_levels", 10))
         p.coarsest_size = int(self._amg_cfg.get("coarsest_size", 200))
+        for key, typ in (("equil_mode", int), ("coarse_pin", int), ("trunc", float), ("pmax", int),
+                         ("coarsening", int)):
+            if key in self._amg_cfg:
+                setattr(p, key, typ(self._amg_cfg[key]))
         self._model.set_amg(p)
         tx = self._model.export("tx", device=self.device).view(nx, ny, nz)
         ty = self._model.export("ty", device=self.device).view(nx, ny, nz)

@@ -13,6 +13,24 @@ The reference's energy-minimising blend of interpolation weights
 (classical_amg.py:389-418) indexes the fine-level basis with coarse column
 numbers; it is not restated (documented deviation).
 
+Four preconditioner choices are not in the reference and default to values
+that differ from it (``REFERENCE`` holds the reference behaviour); they change
+how fast the outer solver converges, not the converged answer:
+  equil_mode  1 = scale the fine operator by 1/median|a_ii| (frozen at setup)
+              instead of the symmetric D^-1/2 A D^-1/2 (mode 0). The symmetric
+              form moves the near-nullspace to D^1/2 * 1 while the row-normalised
+              interpolation reproduces only constants.
+  coarse_pin  2 = pin DOF 0 of the coarsest level only when that operator is
+              singular (every row |row sum| <= 1e-8 |a_ii|); 1 = reference.
+  trunc/pmax  truncation of each F row of P (weights below trunc * row max,
+              then all but the pmax largest), rescaled to the row's sum.
+  coarsening  1 = iterated independent-set C/F split (rounds of the same
+              local-max selection among undecided points); 0 = the reference's
+              single pass with lift, which coarsens poorly on dense deep levels.
+These are pinned by invariants in tests/test_oracle_amg.py (scaling
+invariance, row sums kept, pin decision on Neumann vs. nonsingular operators,
+a maximal independent set checked round by round).
+
 Parity pinning of each function (checked in tests/test_oracle_amg.py):
   strength            brute-force per-row loop on small matrices
   cf_split            brute-force definition (strict local maxima + lift)
@@ -101,6 +119,38 @@ def cf_split(A: sp.csr_matrix, strong: np.ndarray):
     has_c = np.zeros(n, dtype=bool)
     np.logical_or.at(has_c, rows[ing], is_max[A.indices[ing]])
     cf = np.where(is_max, C_PT, np.where(has_c, F_PT, C_PT)).astype(np.int8)
+    return cf, int(np.sum(cf == C_PT))
+
+
+def cf_split_iterated(A: sp.csr_matrix, strong: np.ndarray):
+    """Iterated independent-set split (not in the reference): rounds of the
+    cf_split selection restricted to undecided points; the undecided graph
+    neighbours of each new C point become F. Returns (cf, number of C points)."""
+    n = A.shape[0]
+    rows = _rows(A)
+    tp = transpose_pos(A)
+    if np.any(tp < 0):
+        raise ValueError("matrix pattern is not structurally symmetric")
+    ing = (strong | strong[tp]) & (A.indices != rows)
+    deg = np.bincount(rows[ing], minlength=n)
+    h = hash32(np.arange(n))
+    order = np.lexsort((np.arange(n), h, deg))
+    rank = np.empty(n, dtype=np.int64)
+    rank[order] = np.arange(n)
+    r_e, c_e = rows[ing], A.indices[ing]
+    und = np.ones(n, dtype=bool)
+    cf = np.full(n, F_PT, dtype=np.int8)
+    while und.any():
+        m = und[r_e] & und[c_e]
+        nm = np.full(n, -1, dtype=np.int64)
+        np.maximum.at(nm, r_e[m], rank[c_e[m]])
+        newc = und & (rank > nm)
+        cf[newc] = C_PT
+        und &= ~newc
+        cov = np.zeros(n, dtype=bool)
+        mm = und[r_e] & newc[c_e]
+        cov[r_e[mm]] = True
+        und &= ~cov
     return cf, int(np.sum(cf == C_PT))
 
 
@@ -205,6 +255,42 @@ def interpolation(A: sp.csr_matrix, cf: np.ndarray, strong: np.ndarray) -> sp.cs
     return P.tocsr()
 
 
+def truncate(P: sp.csr_matrix, cf: np.ndarray, trunc: float, pmax: int) -> sp.csr_matrix:
+    """Truncation of the F rows of P (not in the reference). Per F row with more
+    than one weight: tot = sequential sum, drop w < trunc * max(w), keep the pmax
+    largest (ties: lower coarse column), scale the kept weights by tot / their
+    sequential sum. C rows are untouched. Weights within 1e-10 * max(w) of each
+    other count as tied, so ulp-level rounding differences do not change the
+    pattern."""
+    if trunc <= 0.0 and pmax <= 0:
+        return P
+    P = sp.csr_matrix(P)
+    ptr, idx, val = P.indptr, P.indices, P.data
+    rows_out, cols_out, vals_out = [], [], []
+    for i in range(P.shape[0]):
+        s, e = ptr[i], ptr[i + 1]
+        c, v = idx[s:e], val[s:e]
+        if cf[i] == C_PT or e - s <= 1:
+            rows_out.append(np.full(e - s, i)); cols_out.append(c); vals_out.append(v)
+            continue
+        tot = np.cumsum(v)[-1]
+        tie = 1e-10 * v.max()
+        keep = v + tie >= trunc * v.max()
+        c, v = c[keep], v[keep]
+        if pmax > 0 and v.size > pmax:
+            k = v.size
+            above = (v[None, :] > v[:, None] + tie) | (
+                (np.abs(v[None, :] - v[:, None]) <= tie) & (np.arange(k)[None, :] < np.arange(k)[:, None]))
+            sel = above.sum(axis=1) < pmax
+            c, v = c[sel], v[sel]
+        v = v * (tot / np.cumsum(v)[-1])
+        rows_out.append(np.full(v.size, i)); cols_out.append(c); vals_out.append(v)
+    out = sp.csr_matrix((np.concatenate(vals_out), (np.concatenate(rows_out), np.concatenate(cols_out))),
+                        shape=P.shape)
+    out.sort_indices()
+    return out
+
+
 def inv_relax(A: sp.csr_matrix) -> np.ndarray:
     """classical_amg.py:885-889: 1/max(|a_ii|, 0.3*sum_j|a_ij|), capped at 1e2."""
     d = np.abs(_diag_first(A))
@@ -217,20 +303,34 @@ def inv_relax(A: sp.csr_matrix) -> np.ndarray:
 class Hierarchy:
     """classical_amg.py:699-938 (setup) and :1037-1180 (cycles)."""
 
+    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0}
+
     def __init__(self, A0: sp.csr_matrix, theta=0.25, max_levels=10, coarsest_size=200, pre=3, post=2,
-                 omega=0.7, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50):
+                 omega=0.7, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, equil_mode=1, coarse_pin=2,
+                 trunc=0.2, pmax=4, coarsening=1):
         self.theta, self.max_levels, self.coarsest_size = theta, max_levels, coarsest_size
         self.pre, self.post, self.omega = pre, post, omega
         self.equil_threshold, self.dense_max, self.coarse_sweeps = equil_threshold, dense_max, coarse_sweeps
+        self.equil_mode, self.coarse_pin, self.trunc, self.pmax = equil_mode, coarse_pin, trunc, pmax
+        self.coarsening = coarsening
         A0 = sp.csr_matrix(A0, dtype=np.float64)
         A0.sort_indices()
         self._setup(A0)
 
-    def _equilibrate(self, A0):
+    def _equilibrate(self, A0, decide=True):
         d = np.maximum(np.abs(_diag_first(A0)), 1e-30)
+        self.diag0 = _diag_first(A0)
+        if self.equil_mode == 1:
+            if decide:
+                med = np.sort(d)[(d.size - 1) // 2]
+                self.equilibrated = bool(med < self.equil_threshold)
+                self.scalar = 1.0 / np.sqrt(med) if self.equilibrated else 1.0
+            self.dh = np.full(A0.shape[0], self.scalar)
+            A = A0.copy()
+            A.data = A0.data * self.scalar * self.scalar
+            return A
         med = np.sort(d)[(d.size - 1) // 2]  # torch.median: lower median
         self.equilibrated = bool(med < self.equil_threshold)
-        self.diag0 = _diag_first(A0)
         if self.equilibrated:
             self.dh = 1.0 / np.sqrt(d)
             Dh = sp.diags(self.dh)
@@ -249,11 +349,11 @@ class Hierarchy:
             if n <= self.coarsest_size:
                 break
             s = strength(A, self.theta)
-            cf0, ncoarse = cf_split(A, s)
+            cf0, ncoarse = (cf_split_iterated if self.coarsening == 1 else cf_split)(A, s)
             if ncoarse >= 0.9 * n:
                 break
             cf = promote_orphans(A, s, cf0)
-            P = interpolation(A, cf, s)
+            P = truncate(interpolation(A, cf, s), cf, self.trunc, self.pmax)
             R = P.T.tocsr()
             R.sort_indices()
             Ac = (R @ (A @ P)).tocsr()
@@ -265,14 +365,19 @@ class Hierarchy:
         self._coarse_factor()
 
     def _coarse_factor(self):
-        """classical_amg.py:1045-1069: dense solve with DOF 0 pinned. The pin is
-        applied only when the hierarchy has more than one level: alone it would
-        freeze DOF 0 of a nonsingular system (a single-level hierarchy is solved
-        exactly instead)."""
+        """classical_amg.py:1045-1069: dense solve, DOF 0 pinned per coarse_pin.
+        A single-level hierarchy is never pinned: alone the pin would freeze
+        DOF 0 of a nonsingular system."""
         L = self.levels[-1]
         n = L["A"].shape[0]
         L["dense"] = n <= self.dense_max
-        L["pin"] = len(self.levels) > 1
+        mode = self.coarse_pin if len(self.levels) > 1 else 0
+        if mode == 2:
+            A = L["A"]
+            rs = np.asarray(A.sum(axis=1)).ravel()
+            L["pin"] = not bool(np.any(np.abs(rs) > 1e-8 * np.abs(A.diagonal())))
+        else:
+            L["pin"] = mode == 1
         if L["dense"]:
             Ad = L["A"].toarray()
             if L["pin"]:
@@ -285,7 +390,7 @@ class Hierarchy:
         """classical_amg.py:1308-1392: new values, frozen P."""
         A0 = sp.csr_matrix(A0, dtype=np.float64)
         A0.sort_indices()
-        A = self._equilibrate(A0)
+        A = self._equilibrate(A0, decide=False)
         for L in self.levels:
             L["A"] = A
             L["ir"] = inv_relax(A)

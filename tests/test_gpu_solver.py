@@ -91,19 +91,25 @@ def _compare_hierarchy(hg, ho, tol=1e-11):
             assert np.max(np.abs(P.data - L["P"].data)) < 1e-13, f"level {l} P values"
 
 
-@pytest.mark.parametrize("case", ["laplacian", "impes_2d", "impes_3d"])
-def test_hierarchy_matches_oracle(dev, case):
+@pytest.mark.parametrize("mode", ["reference", "default"])
+@pytest.mark.parametrize("case", ["laplacian", "neumann", "impes_2d", "impes_3d"])
+def test_hierarchy_matches_oracle(dev, case, mode):
     import impes_hip as H
 
     if case == "laplacian":
         A = apply_reference_fix(laplacian_2d(12), 0)
         kw = dict(theta=0.25, max_levels=10, coarsest_size=40)
+    elif case == "neumann":  # singular: the coarsest level is pinned in both modes
+        A = laplacian_2d(24)
+        kw = dict(theta=0.25, max_levels=10, coarsest_size=60)
     elif case == "impes_2d":
         A, *_ = impes_like(60, 60, 1, seed=1)
         kw = dict(theta=0.25, max_levels=10, coarsest_size=200)
     else:
         A, *_ = impes_like(20, 20, 10, seed=1)
         kw = dict(theta=0.25, max_levels=10, coarsest_size=200)
+    if mode == "reference":
+        kw.update(oa.Hierarchy.REFERENCE)
     ho = oa.Hierarchy(A, **kw)
     hg = H.AmgHierarchy(**kw)
     hg.setup_csr(*to_dev(A, dev))
