@@ -1,0 +1,172 @@
+"""IMPES throughput on one MI355X: the three-phase gas-injection black-oil case at
+1M cells (100x100x100, dx=dy=10 m, dz=1 m, sigma_ln 1.5, 9 wells, dt = 1 day),
+pressure by FGMRES (FP64) preconditioned with the RS-AMG hierarchy in FP32, to a
+1e-8 relative residual, beside the same run with the FP64 hierarchy.
+
+A step is one IMPES time step: TPFA assembly, AMG numeric update + FGMRES solve,
+explicit upwind transport with its substeps and ledgers. Inputs are resident in
+HBM; the first warmup step builds the AMG hierarchy (pattern frozen afterwards,
+as in the reference's cached solver).
+
+Prints ONE JSON line (rank 0). ``--gpus N`` with N > 1 exits: this build runs on
+one GPU (see the sharding section of the design notes)."""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "oil-simulator-py_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "IMPES cell-steps/s @1M cells, FP32-AMG+FP64-GMRES vs FP64 AMG at 1e-8 rel res"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=18)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dims", default="100x100x100")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-hierarchy comparison run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-dims", default="50x50x40")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    return ap.parse_args()
+
+
+def make_sim(dims, precision):
+    from simulator.cases import baseline_case, build
+
+    cfg, fields = baseline_case(2, dims=dims)
+    cfg["simulation"]["amg"] = {"tol": 1e-8, "precision": precision, "max_levels": 25, "max_iter": 400}
+    return cfg, build(cfg, fields, "cuda")
+
+
+def run(dims, precision, steps, warmup, ktimer=False):
+    import torch
+    import impes_hip as H
+
+    cfg, sim = make_sim(dims, precision)
+    dt = cfg["simulation"]["time_step_days"] * 86400.0
+    for _ in range(warmup):
+        if not sim.run_step(dt):
+            raise RuntimeError("warmup step failed")
+    torch.cuda.synchronize()
+    if ktimer:
+        H.ktimer_reset(enable=True)
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not sim.run_step(dt):
+            raise RuntimeError("timed step failed (pressure solve did not converge)")
+        its.append(int(sim.last_solve.get("iterations", 0)))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kt = H.ktimer_read() if ktimer else None
+    if ktimer:
+        H.ktimer_reset(enable=False)
+    return sim, el, its, kt
+
+
+def cpu_baseline(dims, steps):
+    """The oracle (CPU restatement, same algorithm as the device path: FP64 AMG +
+    FGMRES to 1e-8) on a bounded sample of the workload: a dims-sized grid of the
+    same case (same field generator, wells placed proportionally), one untimed
+    step that builds the hierarchy, then `steps` timed steps."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import amg as oa, impes as oi
+    from simulator.cases import baseline_case
+
+    cfg, fields = baseline_case(2, dims=dims)
+    case = oi.Case(cfg, fields, pvt=oi.PVT(cfg["fluid"]["pvt_table"]))
+    sim = oi.Simulation(case)
+    state = {}
+
+    def solver(A, b):
+        if "H" not in state:
+            state["H"] = oa.Hierarchy(A, max_levels=25)
+        else:
+            state["H"].update(A)
+        x, st = oa.fgmres(A, b, x0=None, precond=state["H"].precondition, rtol=1e-8, maxit=400)
+        return x
+
+    dt = cfg["simulation"]["time_step_days"] * 86400.0
+    sim.step(dt, solver=solver)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(dt, solver=solver)
+    el = time.perf_counter() - t0
+    n = int(np.prod(dims))
+    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (numpy/scipy, 1 thread) on a {dims[0]}x{dims[1]}x{dims[2]} grid of the same case, "
+                      f"{steps} steps after one hierarchy-building step, {el:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is described in the design notes, not implemented)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import impes_hip as H
+
+    H.load()  # fails loudly when the HIP library is missing
+    dims = tuple(int(x) for x in a.dims.split("x"))
+    ncell = int(np.prod(dims))
+
+    sim32, el32, its32, kt = run(dims, "fp32", a.steps, a.warmup, ktimer=True)
+    value = ncell * a.steps / el32
+    p32 = sim32.fluid.pressure.clone()
+    levels = sim32.amg_levels()
+
+    extra = {}
+    if not a.no_fp64:
+        sim64, el64, its64, _ = run(dims, "fp64", a.steps, a.warmup)
+        p64 = sim64.fluid.pressure
+        extra = {"fp64_value": ncell * a.steps / el64, "fp64_ms_per_step": 1e3 * el64 / a.steps,
+                 "speedup_fp32_vs_fp64": el64 / el32,
+                 "fp32_vs_fp64_pressure_max_rel": float((p32 - p64).abs().max() / p64.abs().max()),
+                 "fgmres_iterations_fp64": its64}
+        del sim64
+
+    roof = None
+    if kt is not None and kt["launches"] > 0:
+        per_launch_s = kt["ms"] * 1e-3 / kt["launches"]
+        bytes_per_launch = kt["bytes"] / kt["launches"]
+        achieved = bytes_per_launch / per_launch_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get(kt["name"], {}).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kt["name"],
+                "launches": kt["launches"], "avg_us": per_launch_s * 1e6,
+                "algorithmic_bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if not a.no_cpu:
+        cdims = tuple(int(x) for x in a.cpu_dims.split("x"))
+        cpu = cpu_baseline(cdims, a.cpu_steps)
+
+    out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * el32 / a.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 (FP32 AMG preconditioner)", "data": "synthetic",
+           "config": {"workload": "three-phase gas-injection black-oil IMPES, 100x100x100 (1M cells)",
+                      "dims": list(dims), "cells": ncell, "dt_days": 1.0, "wells": 9, "rel_residual": 1e-8,
+                      "amg_levels": [list(x) for x in levels], "fgmres_iterations": its32,
+                      "parallelism": "single GPU"},
+           "roofline": roof, "cpu_baseline": cpu}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

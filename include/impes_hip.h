@@ -123,6 +123,10 @@ int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, 
 /* Preconditioner of the row-scaled system D^-1 A: z = S V(S D v), S = D^-1/2 or I.
  * Replaces the apply closure of amg_preconditioner, src/solver/pressure_amg.py:155-158. */
 int impes_amg_precondition(impes_amg* amg, int precision, const double* v, double* z, void* stream);
+/* No counterpart in the reference: precondition() replays the V-cycle from a HIP graph
+ * captured on first use after each setup (default; environment IMPES_GRAPHS=0 turns it
+ * off for new hierarchies). enable 0 runs the same kernels as eager launches. */
+int impes_amg_set_graphs(impes_amg* amg, int enable);
 
 /* ------------------------------------------------------------------ FGMRES */
 
@@ -240,6 +244,16 @@ int impes_model_eval_props(impes_model* m, long long n, const double* p, const d
 int impes_model_eval_pvt(impes_model* m, int key, long long n, const double* p, double* out, void* stream);
 /* Hierarchy summary of the pressure solver (after the first solve). */
 int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_host, int* nnz_host, int max_levels);
+
+/* ------------------------------------------------------------------ measurement */
+/* No counterpart in the reference (instrumentation). Times every launch of one
+ * kernel, by name ("st7_jacobi_f32", "st7_jacobi_f64", "csr_jacobi_f32",
+ * "csr_jacobi_f64", "csr_residual_f32", "st7_residual_f32",
+ * "st7_rowscaled_spmv_f64"), with a HIP event pair on its launching stream, and
+ * adds the algorithmic bytes of each launch. config resets the totals; kernel
+ * NULL or enable 0 switches timing off. read waits for the recorded events. */
+int impes_ktimer_config(const char* kernel, int enable);
+int impes_ktimer_read(double* ms, double* bytes, long long* launches);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,10 @@ struct Level {
 
 class Hierarchy {
  public:
-  explicit Hierarchy(const AmgParams& p) : prm_(p) {}
+  explicit Hierarchy(const AmgParams& p);
+  ~Hierarchy();
+  Hierarchy(const Hierarchy&) = delete;
+  Hierarchy& operator=(const Hierarchy&) = delete;
 
   // Full setup from an (unscaled) CSR matrix with sorted column indices.
   void setup_csr(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s);
@@ -105,7 +108,11 @@ class Hierarchy {
 
   // Preconditioner of the row-scaled system D^-1 A x = D^-1 b:
   // z = S .* V(S .* D .* v)  with S = D^-1/2 (equilibrated) or I.
+  // The V-cycle between the two scalings is replayed from a HIP graph captured on
+  // first use after each setup (its buffers are fixed; updates keep them).
   void precondition(int precision, const double* v, double* z, hipStream_t s);
+  void set_graphs(bool on) { graphs_ = on; }
+  bool graphs() const { return graphs_; }
 
   // Stationary V-cycle iteration with physical-space input/output, the semantics of
   // ClassicalAMG.solve (classical_amg.py:1108-1155) and .apply (:1157-1180):
@@ -141,6 +148,12 @@ class Hierarchy {
   void smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& zero, hipStream_t s);
   template <class T>
   void coarse_solve(int l, const T* b, T* x, hipStream_t s);
+  void drop_graphs();
+  void capture_vcycle(int precision);
+
+  bool graphs_ = true;
+  hipStream_t cap_stream_ = nullptr;       // capture only; replays run on the caller's stream
+  hipGraphExec_t vexec_[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64 V-cycle
 
   AmgParams prm_;
   std::vector<Level> levels_;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "amg_kernels.h"
+#include "ktimer.h"
 #include "prims.h"
 
 namespace impes {
@@ -189,9 +190,49 @@ void Hierarchy::setup_csr(int n, int nnz, const int* rp, const int* ci, const do
   setup_impl(n, nnz, rp, ci, v, s);
 }
 
+Hierarchy::Hierarchy(const AmgParams& p) : prm_(p) {
+  // IMPES_GRAPHS=0 runs every V-cycle as eager launches
+  const char* g = std::getenv("IMPES_GRAPHS");
+  graphs_ = !(g && g[0] == '0');
+}
+
+Hierarchy::~Hierarchy() {
+  drop_graphs();
+  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+void Hierarchy::drop_graphs() {
+  for (auto& e : vexec_) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
+}
+
+void Hierarchy::capture_vcycle(int precision) {
+  if (!cap_stream_) IMPES_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+  Level& L0 = levels_[0];
+  hipGraph_t g = nullptr;
+  IMPES_HIP_CHECK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    if (precision == 32) vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, cap_stream_);
+    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, cap_stream_);
+  } catch (...) {
+    (void)hipStreamEndCapture(cap_stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  IMPES_HIP_CHECK(hipStreamEndCapture(cap_stream_, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  IMPES_HIP_CHECK(rc);
+  vexec_[precision == 32 ? 0 : 1] = e;
+}
+
 void Hierarchy::setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && nnz >= 1, "amg setup: empty matrix");
   const double t0 = wall();
+  drop_graphs();  // the level buffers are about to be rebuilt
   levels_.clear();
   levels_.emplace_back();
   Level& L0 = levels_[0];
@@ -683,13 +724,18 @@ int Hierarchy::cycles(int precision, const double* b, double* x, bool use_x0, in
 void Hierarchy::precondition(int precision, const double* v, double* z, hipStream_t s) {
   Level& L0 = levels_[0];
   const long long n = L0.n;
+  const bool replay = graphs_ && !ktimer_enabled();
+  const int gi = precision == 32 ? 0 : 1;
+  if (replay && !vexec_[gi]) capture_vcycle(precision);
   if (precision == 32) {
     launch_scale_f64_to_f32(L0.b32.p, sin_.p, v, n, s);
-    vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
+    if (replay) IMPES_HIP_CHECK(hipGraphLaunch(vexec_[gi], s));
+    else vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
     launch_scale_f32_to_f64(z, sout_.p, L0.x32.p, n, s);
   } else {
     launch_scale_f64(L0.b64.p, sin_.p, v, n, s);
-    vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
+    if (replay) IMPES_HIP_CHECK(hipGraphLaunch(vexec_[gi], s));
+    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
     launch_scale_f64(z, sout_.p, L0.x64.p, n, s);
   }
 }

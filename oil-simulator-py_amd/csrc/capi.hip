@@ -9,6 +9,7 @@
 #include "amg_kernels.h"
 #include "fgmres.h"
 #include "impes_model.h"
+#include "ktimer.h"
 
 namespace impes {
 namespace {
@@ -303,6 +304,13 @@ int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, 
   });
 }
 
+int impes_amg_set_graphs(impes_amg* amg, int enable) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(amg, "null pointer");
+    amg->h.set_graphs(enable != 0);
+  });
+}
+
 int impes_amg_precondition(impes_amg* amg, int precision, const double* v, double* z, void* stream) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(amg && v && z, "null pointer");
@@ -549,6 +557,14 @@ int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_hos
       if (nnz_host) nnz_host[l] = li.nnz;
     }
   });
+}
+
+int impes_ktimer_config(const char* kernel, int enable) {
+  return guarded(__func__, [&] { ktimer_config(kernel, enable != 0); });
+}
+
+int impes_ktimer_read(double* ms, double* bytes, long long* launches) {
+  return guarded(__func__, [&] { ktimer_read(nullptr, ms, bytes, launches); });
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 // so it is stored index-free as one diagonal and three face-coefficient arrays
 // (4 values per cell instead of CSR's ~7 values + 7 indices + row pointers).
 #include "kernels.h"
+#include "ktimer.h"
 
 namespace impes {
 
@@ -117,6 +118,19 @@ void st7_launch(const Stencil7<T>& S, const T* x, const T* b, const TA* aux, T w
 
 }  // namespace
 
+// Algorithmic HBM bytes of one pass (each operand read or written once):
+// CSR: row pointers + (column, value) per entry + `vecs` row vectors besides x
+// (x read once, e.g. b, inv_relax and the output for Jacobi);
+// stencil: four coefficient arrays + x + `vecs` row vectors.
+double csr_bytes(long long n, long long nnz, int val_bytes, int vecs) {
+  return (double)(n + 1) * 4 + (double)nnz * (4 + val_bytes) + (double)n * val_bytes * (1 + vecs);
+}
+template <class T>
+double st7_bytes(const Stencil7<T>& S, int val_bytes, int vecs) {
+  const double n = (double)S.nx * S.ny * S.nz;
+  return n * val_bytes * (4 + 1 + vecs);
+}
+
 int csr_group_for(int n_rows, int nnz) {
   if (n_rows <= 0) return 1;
   const double mean = (double)nnz / (double)n_rows;
@@ -141,14 +155,17 @@ void launch_residual_f64(const CsrF64& A, const double* x, const double* b, doub
   csr_dispatch<double, double, kRes>(A.n_rows, A.rp, A.ci, A.v, x, b, (const double*)nullptr, 0.0, r, G, s);
 }
 void launch_residual_f32(const CsrF32& A, const float* x, const float* b, float* r, int G, hipStream_t s) {
+  KScope ks("csr_residual_f32", csr_bytes(A.n_rows, A.nnz, 4, 2), s);
   csr_dispatch<float, float, kRes>(A.n_rows, A.rp, A.ci, A.v, x, b, (const float*)nullptr, 0.f, r, G, s);
 }
 void launch_jacobi_f64(const CsrF64& A, const double* xi, const double* b, const double* inv_relax, double w,
                        double* xo, int G, hipStream_t s) {
+  KScope ks("csr_jacobi_f64", csr_bytes(A.n_rows, A.nnz, 8, 3), s);
   csr_dispatch<double, double, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
 }
 void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const float* inv_relax, float w, float* xo,
                        int G, hipStream_t s) {
+  KScope ks("csr_jacobi_f32", csr_bytes(A.n_rows, A.nnz, 4, 3), s);
   csr_dispatch<float, float, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
 }
 void launch_jacobi0_f64(const double* b, const double* inv_relax, double w, double* xo, int n, hipStream_t s) {
@@ -177,18 +194,22 @@ void launch_st7_residual_f64(const Stencil7<double>& S, const double* x, const d
   st7_launch<double, double, kRes>(S, x, b, (const double*)nullptr, 0.0, r, s);
 }
 void launch_st7_residual_f32(const Stencil7<float>& S, const float* x, const float* b, float* r, hipStream_t s) {
+  KScope ks("st7_residual_f32", st7_bytes(S, 4, 2), s);
   st7_launch<float, float, kRes>(S, x, b, (const float*)nullptr, 0.f, r, s);
 }
 void launch_st7_jacobi_f64(const Stencil7<double>& S, const double* xi, const double* b, const double* inv_relax,
                            double w, double* xo, hipStream_t s) {
+  KScope ks("st7_jacobi_f64", st7_bytes(S, 8, 3), s);
   st7_launch<double, double, kJac>(S, xi, b, inv_relax, w, xo, s);
 }
 void launch_st7_jacobi_f32(const Stencil7<float>& S, const float* xi, const float* b, const float* inv_relax, float w,
                            float* xo, hipStream_t s) {
+  KScope ks("st7_jacobi_f32", st7_bytes(S, 4, 3), s);
   st7_launch<float, float, kJac>(S, xi, b, inv_relax, w, xo, s);
 }
 void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv, const double* x, double* y,
                                    hipStream_t s) {
+  KScope ks("st7_rowscaled_spmv_f64", st7_bytes(S, 8, 2), s);
   st7_launch<double, double, kRowScaled>(S, x, nullptr, dinv, 0.0, y, s);
 }
 void launch_st7_rowscaled_residual_f64(const Stencil7<double>& S, const double* dinv, const double* x,

@@ -122,6 +122,7 @@ SIGNATURES = {
     "impes_amg_export_scaling": [_P, _P, _P],
     "impes_amg_vcycle": [_P, _I, _P, _P, _I, _P],
     "impes_amg_precondition": [_P, _I, _P, _P, _P],
+    "impes_amg_set_graphs": [_P, _I],
     "impes_solver_create": [C.POINTER(_P)],
     "impes_solver_destroy": [_P],
     "impes_fgmres_csr": [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I, C.POINTER(SolveStats), _P],
@@ -143,6 +144,8 @@ SIGNATURES = {
     "impes_model_eval_props": [_P, _L, _P, _P, _P, _P, _P],
     "impes_model_eval_pvt": [_P, _I, _L, _P, _P, _P],
     "impes_model_amg_levels": [_P, _PI, _PI, _PI, _I],
+    "impes_ktimer_config": [C.c_char_p, _I],
+    "impes_ktimer_read": [_PD, _PD, C.POINTER(_L)],
 }
 _RESTYPE = {"impes_last_error": C.c_char_p, "impes_amg_default_params": None}
 

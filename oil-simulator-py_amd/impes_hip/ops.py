@@ -13,6 +13,7 @@ from ._lib import AmgParams, ImpesError, SolveStats, call, load, ptr, stream_ptr
 
 __all__ = [
     "spmv_csr", "spmv_st7", "l1jacobi_csr", "dot", "AmgHierarchy", "FgmresSolver", "amg_default_params",
+    "ktimer_reset", "ktimer_read",
     "ImpesModel", "csr_diagonal",
 ]
 
@@ -64,6 +65,22 @@ def csr_diagonal(rowptr, col, val):
     out = torch.empty(n, dtype=torch.float64, device=val.device)
     call("impes_csr_diagonal", n, ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(out), stream_ptr())
     return out
+
+
+_KT = {"name": ""}
+
+
+def ktimer_reset(enable: bool = True, kernel: str = "st7_jacobi_f32"):
+    """Select the kernel whose launches are timed with HIP event pairs (resets totals)."""
+    _KT["name"] = kernel if enable else ""
+    call("impes_ktimer_config", kernel.encode() if enable else None, 1 if enable else 0)
+
+
+def ktimer_read() -> dict:
+    """Totals since ktimer_reset: event-timed milliseconds, algorithmic bytes, launches."""
+    ms, by, n = C.c_double(), C.c_double(), C.c_longlong()
+    call("impes_ktimer_read", C.byref(ms), C.byref(by), C.byref(n))
+    return {"name": _KT["name"], "ms": ms.value, "bytes": by.value, "launches": n.value}
 
 
 def amg_default_params() -> AmgParams:
@@ -196,6 +213,10 @@ class AmgHierarchy:
             x = torch.empty_like(b)
         call("impes_amg_vcycle", self._h, int(precision), ptr(b), ptr(x), 1 if zero else 0, stream_ptr())
         return x
+
+    def set_graphs(self, enable: bool):
+        """V-cycle replay from a captured HIP graph (default) or eager launches."""
+        call("impes_amg_set_graphs", self._h, 1 if enable else 0)
 
     def precondition(self, v, precision=64, out=None):
         z = out if out is not None else torch.empty_like(v)

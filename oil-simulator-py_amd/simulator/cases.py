@@ -10,7 +10,8 @@ placed proportionally) for parity tests.
   1  three-phase 100x100x20, layered anisotropic correlated k, PVT, Stone II,
      capillarity, five-spot water injection, gravity
   2  three-phase gas injection 100x100x100 (1M cells), sigma_ln 1.5, 9 wells
-  3  two-phase 64x64x8, correlated log-k (truth field of the history match)
+  3  two-phase 64x64x8, correlated log-k (truth field of the history match),
+     2 water injectors at 100 m^3/day and 3 liquid producers at 60 m^3/day
   4  = 2 (forward problem of the adjoint at scale)
 
 The capillary pressure "Brooks-Corey, lambda = 2, entry 10 kPa" is realised
@@ -158,10 +159,12 @@ def baseline_case(index: int, dims=None, steps=None):
                                control_value=100.0),
                          _well("INJ-2", "injector", (55, 55, 4), full, dims, control_type="rate",
                                control_value=100.0),
-                         _well("PROD-1", "producer", (55, 8, 4), full, dims, control_type="bhp", control_value=15.0),
-                         _well("PROD-2", "producer", (8, 55, 4), full, dims, control_type="bhp", control_value=15.0),
-                         _well("PROD-3", "producer", (32, 32, 4), full, dims, control_type="bhp",
-                               control_value=15.0)],
+                         _well("PROD-1", "producer", (55, 8, 4), full, dims, control_type="rate",
+                               control_value=60.0),
+                         _well("PROD-2", "producer", (8, 55, 4), full, dims, control_type="rate",
+                               control_value=60.0),
+                         _well("PROD-3", "producer", (32, 32, 4), full, dims, control_type="rate",
+                               control_value=60.0)],
                "simulation": {"solver_type": "impes", "time_step_days": 5.0, "steps": steps or 30,
                               "pressure_solver": "amg", "amg": {"tol": 1e-8}}}
         return cfg, fields
