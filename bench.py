@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-dims", default="50x50x40")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--kernel", default="st7_jacobi_f32", help="kernel timed for the roofline figure")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (FP64) and exit: the known-size kernel that "
+                         "calibrates FETCH_SIZE/WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -46,7 +50,10 @@ def make_sim(dims, precision):
     return cfg, build(cfg, fields, "cuda")
 
 
-def run(dims, precision, steps, warmup, ktimer=False):
+def run(dims, precision, steps, warmup, kernel_steps=0, kernel="st7_jacobi_f32"):
+    """warmup + `steps` timed steps (no instrumentation), then `kernel_steps` more
+    steps with every launch of `kernel` bracketed by HIP events on its stream
+    (V-cycle graph replay is bypassed while a kernel is timed)."""
     import torch
     import impes_hip as H
 
@@ -56,8 +63,6 @@ def run(dims, precision, steps, warmup, ktimer=False):
         if not sim.run_step(dt):
             raise RuntimeError("warmup step failed")
     torch.cuda.synchronize()
-    if ktimer:
-        H.ktimer_reset(enable=True)
     its = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -66,8 +71,14 @@ def run(dims, precision, steps, warmup, ktimer=False):
         its.append(int(sim.last_solve.get("iterations", 0)))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    kt = H.ktimer_read() if ktimer else None
-    if ktimer:
+    kt = None
+    if kernel_steps:
+        H.ktimer_reset(enable=True, kernel=kernel)
+        for _ in range(kernel_steps):
+            if not sim.run_step(dt):
+                raise RuntimeError("kernel-timing step failed")
+        torch.cuda.synchronize()
+        kt = H.ktimer_read()
         H.ktimer_reset(enable=False)
     return sim, el, its, kt
 
@@ -119,13 +130,22 @@ def main():
     import impes_hip as H
 
     H.load()  # fails loudly when the HIP library is missing
+    if a.calibrate_copy:
+        n = a.calibrate_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            H.copy_f64(y, x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 8, "calibrate_copy_bytes_written": n * 8}), flush=True)
+        return
     dims = tuple(int(x) for x in a.dims.split("x"))
     ncell = int(np.prod(dims))
 
-    sim32, el32, its32, kt = run(dims, "fp32", a.steps, a.warmup, ktimer=True)
+    sim32, el32, its32, _ = run(dims, "fp32", a.steps, a.warmup)
     value = ncell * a.steps / el32
     p32 = sim32.fluid.pressure.clone()
-    levels = sim32.amg_levels()
+    del sim32
 
     extra = {}
     if not a.no_fp64:
@@ -137,6 +157,10 @@ def main():
                  "fgmres_iterations_fp64": its64}
         del sim64
 
+    # roofline of the dominant kernel, on a fresh run of the same workload
+    sim_k, _, _, kt = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernel=a.kernel)
+    levels = sim_k.amg_levels()
+    del sim_k
     roof = None
     if kt is not None and kt["launches"] > 0:
         per_launch_s = kt["ms"] * 1e-3 / kt["launches"]

@@ -64,6 +64,10 @@ int impes_csr_diagonal(int n, const int* rowptr, const int* col, const double* v
  * src/solver/classical_amg.py:1136-1142, src/simulator/simulation.py:1677-1689. */
 int impes_dot_f64(long long n, const double* x, const double* y, double* out, void* stream);
 
+/* y = x (device, n doubles). No counterpart in the reference: the known-size
+ * streaming kernel that calibrates HBM traffic counters. */
+int impes_copy_f64(long long n, const double* x, double* y, void* stream);
+
 /* ------------------------------------------------------------------ AMG hierarchy */
 
 typedef struct {

@@ -132,6 +132,13 @@ int impes_csr_diagonal(int n, const int* rowptr, const int* col, const double* v
   });
 }
 
+int impes_copy_f64(long long n, const double* x, double* y, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(n >= 0 && (n == 0 || (x && y)), "bad copy arguments");
+    launch_copy_f64(y, x, n, as_stream(stream));
+  });
+}
+
 int impes_dot_f64(long long n, const double* x, const double* y, double* out, void* stream) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(n >= 0 && x && y && out, "bad argument");

@@ -3,6 +3,8 @@
 // one scalar per Arnoldi step to decide termination.
 #include "fgmres.h"
 
+#include <cstdlib>
+
 #include <cmath>
 
 namespace impes {
@@ -76,12 +78,21 @@ __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
 
 }  // namespace
 
+Fgmres::Fgmres() {
+  const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
+  lookahead_ = !(e && e[0] == '0');
+}
+
 Fgmres::~Fgmres() {
   if (host_) (void)hipHostFree(host_);
+  for (auto& e : est_ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 void Fgmres::ensure(long long n, int m) {
   if (!host_) IMPES_HIP_CHECK(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
+  for (auto& e : est_ev_)
+    if (!e) IMPES_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (n == n_ && m == m_) return;
   n_ = n;
   m_ = m;
@@ -146,6 +157,17 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
     IMPES_LAUNCH_CHECK();
     launch_scale_by_inv_f64(V, V, sc + 0, n, s);
     int k = 0;
+    const int its0 = its;
+    // estimate of iteration jj: async copy into a pinned slot + event, waited on later
+    auto post = [&](int jj) {
+      IMPES_HIP_CHECK(hipMemcpyAsync(host_ + 2 + (jj & 1), sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+      IMPES_HIP_CHECK(hipEventRecord(est_ev_[jj & 1], s));
+    };
+    auto done = [&](int jj) {  // iteration jj (0-based) ends the cycle?
+      IMPES_HIP_CHECK(hipEventSynchronize(est_ev_[jj & 1]));
+      const double est = host_[2 + (jj & 1)];
+      return !(est > tol) || its0 + jj + 1 >= maxit;
+    };
     for (int j = 0; j < m; ++j) {
       double* vj = V + (long long)j * n;
       double* zj = Z + (long long)j * n;
@@ -160,11 +182,22 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
       k_arnoldi<<<1, 1, 0, s>>>(L, j, sm);
       IMPES_LAUNCH_CHECK();
       launch_scale_by_inv_f64(V + (long long)(j + 1) * n, w_.p, sc + 3, n, s);
-      ++its;
+      post(j);
+      if (!lookahead_) {
+        k = j + 1;
+        if (done(j)) break;
+        continue;
+      }
+      if (j > 0 && done(j - 1)) {  // iteration j was speculative
+        k = j;
+        break;
+      }
       k = j + 1;
-      const double est = read(sc + 2);
-      if (!(est > tol) || its >= maxit) break;
+      if (j == m - 1) {
+        (void)done(j);  // the cycle ends here anyway; wait for the copy slot
+      }
     }
+    its = its0 + k;
     k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
     IMPES_LAUNCH_CHECK();
     launch_lincomb_add_f64(x, Z, n, k, sm + L.y(), n, s);

@@ -106,6 +106,7 @@ SIGNATURES = {
     "impes_l1jacobi_csr_f64": [_I, _I, _P, _P, _P, _P, _D, _P, _P, _P, _P],
     "impes_l1jacobi_csr_f32": [_I, _I, _P, _P, _P, _P, _F, _P, _P, _P, _P],
     "impes_dot_f64": [_L, _P, _P, _P, _P],
+    "impes_copy_f64": [_L, _P, _P, _P],
     "impes_csr_diagonal": [_I, _P, _P, _P, _P, _P],
     "impes_amg_default_params": [C.POINTER(AmgParams)],
     "impes_amg_create": [C.POINTER(AmgParams), C.POINTER(_P)],

@@ -13,7 +13,7 @@ from ._lib import AmgParams, ImpesError, SolveStats, call, load, ptr, stream_ptr
 
 __all__ = [
     "spmv_csr", "spmv_st7", "l1jacobi_csr", "dot", "AmgHierarchy", "FgmresSolver", "amg_default_params",
-    "ktimer_reset", "ktimer_read",
+    "ktimer_reset", "ktimer_read", "copy_f64",
     "ImpesModel", "csr_diagonal",
 ]
 
@@ -65,6 +65,13 @@ def csr_diagonal(rowptr, col, val):
     out = torch.empty(n, dtype=torch.float64, device=val.device)
     call("impes_csr_diagonal", n, ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(out), stream_ptr())
     return out
+
+
+def copy_f64(y, x):
+    """y = x with the library's streaming copy kernel (FP64, same length)."""
+    assert x.numel() == y.numel() and x.dtype == y.dtype == torch.float64
+    call("impes_copy_f64", x.numel(), ptr(x), ptr(y), stream_ptr())
+    return y
 
 
 _KT = {"name": ""}

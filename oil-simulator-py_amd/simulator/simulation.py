@@ -212,6 +212,70 @@ class Simulator:
         if self.verbose:
             print(f"P mean = {float(P_new.mean()) / 1e6:.2f} MPa, substeps N={self.last_substeps}", flush=True)
 
+    # ------------------------------------------------------------ run
+    def run(self, output_filename, save_vtk=False, results_root="results"):
+        """simulation.py:2120-2248: total_time_days / time_step_days steps, then the
+        run's artifacts in results_root/<name>_<timestamp>/: <name>.txt (the same
+        SUMMARY and MASS BALANCE lines; full arrays when write_full_arrays_txt) and
+        <name>.npz (pressure, sw, sg, so; when save_npz, default True). Every
+        save_interval steps the fields go to intermediate/<name>_step_<i>.npz
+        (the reference renders PNG plots there; plotting is not part of this
+        package, nor is VTK output). Returns the results directory."""
+        import datetime
+        import os
+
+        import numpy as np
+
+        total_days = self.sim_params.get("total_time_days", 100)
+        step_days = self.sim_params.get("time_step_days", 1.0)
+        save_interval = self.sim_params.get("save_interval", 10)
+        num_steps = int(total_days / step_days)
+        dt = step_days * 86400
+        stamp = datetime.datetime.now().strftime("%Y%m%d_%H%M%S")
+        results_dir = os.path.join(results_root, f"{output_filename}_{stamp}")
+        os.makedirs(results_dir, exist_ok=True)
+        inter = os.path.join(results_dir, "intermediate")
+        if save_interval < num_steps:
+            os.makedirs(inter, exist_ok=True)
+        fl = self.fluid
+
+        def fields():
+            return (fl.pressure.double().cpu().numpy(), fl.s_w.double().cpu().numpy(),
+                    fl.s_g.double().cpu().numpy(), fl.s_o.double().cpu().numpy())
+
+        for i in range(num_steps):
+            ok = self.run_step(dt)
+            if not ok and self.verbose:
+                print(f"  step {i + 1}: pressure solve did not converge")
+            if (i + 1) % save_interval == 0 and save_interval < num_steps:
+                p, sw, sg, so = fields()
+                np.savez_compressed(os.path.join(inter, f"{output_filename}_step_{i + 1}.npz"), pressure=p, sw=sw,
+                                    sg=sg, so=so, day=(i + 1) * step_days)
+        p, sw, sg, so = fields()
+        with open(os.path.join(results_dir, f"{output_filename}.txt"), "w") as f:
+            f.write("=== SUMMARY (final) ===\n")
+            f.write(f"Pressure MPa: min={p.min()/1e6:.2f}, max={p.max()/1e6:.2f}, mean={p.mean()/1e6:.2f}\n")
+            f.write(f"Sw: min={sw.min():.4f}, max={sw.max():.4f}, mean={sw.mean():.4f}\n")
+            f.write(f"Sg: min={sg.min():.4f}, max={sg.max():.4f}, mean={sg.mean():.4f}\n")
+            f.write(f"So: min={so.min():.4f}, max={so.max():.4f}, mean={so.mean():.4f}\n")
+            mb = self.mass_balance
+            net = {ph: mb[ph]["in"] - mb[ph]["out"] for ph in ("water", "gas", "oil")}
+            f.write("\n=== MASS BALANCE (kg) ===\n")
+            f.write(f"water: in={mb['water']['in']:.6e}, out={mb['water']['out']:.6e}, net={net['water']:.6e}\n")
+            f.write(f"gas:   in={mb['gas']['in']:.6e}, out={mb['gas']['out']:.6e}, net={net['gas']:.6e}\n")
+            f.write(f"oil:   in={mb['oil']['in']:.6e}, out={mb['oil']['out']:.6e}, net={net['oil']:.6e}\n")
+            f.write(f"total net={net['water'] + net['gas'] + net['oil']:.6e}\n")
+            if self.sim_params.get("write_full_arrays_txt", False):
+                for title, arr, fmt in (("Final Pressure (MPa)", p / 1e6, "%.2f"), ("Final Water Saturation", sw, "%.4f"),
+                                        ("Final Gas Saturation", sg, "%.4f"), ("Final Oil Saturation", so, "%.4f")):
+                    f.write(f"\n\n{title}:\n")
+                    f.write(np.array2string(arr, threshold=np.inf, formatter={"float_kind": lambda x, fm=fmt: fm % x}))
+        if self.sim_params.get("save_npz", True):
+            np.savez_compressed(os.path.join(results_dir, f"{output_filename}.npz"), pressure=p, sw=sw, sg=sg, so=so)
+        if save_vtk:
+            print("VTK output is not part of this package")
+        return results_dir
+
     # ------------------------------------------------------------ diagnostics
     def volume_balance_error(self) -> dict:
         """Cumulative |sum PV dS - dt sum q - clamp| per phase over the total pore volume."""

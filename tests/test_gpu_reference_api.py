@@ -284,3 +284,25 @@ def test_pvt_rv_loading_and_fluid_rho_g(tmp_path):
                      "k_vertical_fraction": 1.0}, torch.device("cuda"))
     fl = Fluid({"pressure": 40.0, "s_w": 0.2, "s_g": 0.1, "pvt_path": path}, res, torch.device("cuda"))
     assert torch.isfinite(fl.rho_g).all() and (fl.rho_g > 0).all()
+
+
+def test_run_writes_the_reference_artifacts(tmp_path):
+    """simulation.py:2120-2248: run() -> <name>.txt summary + mass balance, <name>.npz fields."""
+    import re
+
+    cfg = gas3_config()
+    cfg["simulation"].update({"solver_type": "impes", "total_time_days": 4, "time_step_days": 1.0,
+                              "save_interval": 2})
+    _, _, fl, sim = mk(cfg)
+    out = sim.run("case", results_root=str(tmp_path))
+    txt = open(os.path.join(out, "case.txt")).read()
+    assert txt.startswith("=== SUMMARY (final) ===\nPressure MPa: min=")
+    assert re.search(r"^water: in=\S+, out=\S+, net=\S+$", txt, re.M)
+    assert re.search(r"^total net=\S+$", txt, re.M)
+    z = np.load(os.path.join(out, "case.npz"))
+    assert set(z.files) == {"pressure", "sw", "sg", "so"}
+    assert np.array_equal(z["pressure"], fl.pressure.double().cpu().numpy())
+    assert sorted(os.listdir(os.path.join(out, "intermediate"))) == ["case_step_2.npz"]
+    mb = sim.mass_balance
+    net = sum(mb[ph]["in"] - mb[ph]["out"] for ph in ("water", "gas", "oil"))
+    assert f"total net={net:.6e}" in txt
