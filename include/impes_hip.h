@@ -83,12 +83,13 @@ typedef struct {
   int anchor;            /* forced C point on every level, -1 = none (reference anchor_idx) */
   /* Preconditioner choices that are not in the reference; they change the iteration
    * count, not the converged answer. The reference behaviour is equil_mode 0,
-   * coarse_pin 1, trunc 0, pmax 0, coarsening 0. */
+   * coarse_pin 1, trunc 0, pmax 0, coarsening 0, dominant 0. */
   int equil_mode;        /* 0 symmetric D^-1/2 A D^-1/2 (reference), 1 scalar A/median|a_ii| (default) */
   int coarse_pin;        /* coarsest DOF-0 pin: 0 never, 1 reference (multi-level), 2 only if singular (default) */
   double trunc;          /* drop interpolation weights below trunc * row max (default 0.2, 0 = off) */
   int pmax;              /* at most pmax interpolation weights per F row (default 4, 0 = unlimited) */
   int coarsening;        /* C/F split: 0 one pass + lift (reference), 1 iterated independent set (default) */
+  double dominant;       /* rows with |a_ii| > dominant * sum|a_ij| become C points (default 10, 0 = off) */
 } impes_amg_params;
 
 void impes_amg_default_params(impes_amg_params* p);

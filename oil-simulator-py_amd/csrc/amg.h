@@ -49,6 +49,10 @@ struct AmgParams {
   double trunc = 0.2;             // drop F-row weights below trunc * row max (0 = keep all)
   int pmax = 4;                   // keep at most pmax weights per F row (0 = unlimited)
   int coarsening = kCoarsenIterated;
+  // rows with |a_ii| > dominant * sum_j!=i |a_ij| (BHP well cells: Dirichlet-like rows)
+  // become C points, so their diagonal stays on one coarse unknown instead of being
+  // spread by the row-normalised interpolation (0 = off, as in the reference)
+  double dominant = 10.0;
 };
 
 struct LevelInfo {
@@ -142,8 +146,9 @@ class Hierarchy {
   void refresh_f32(hipStream_t s);
   void refresh_stencil(hipStream_t s);
   void alloc_vectors();
+  // first_sweep_done: x already holds the first pre-smoothing sweep from zero
   template <class T>
-  void vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s);
+  void vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s, bool first_sweep_done = false);
   template <class T>
   void smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& zero, hipStream_t s);
   template <class T>

@@ -131,6 +131,8 @@ struct Sel<double> {
     launch_spmv_add_f64(A, x, y, g, s);
   }
   static void copy(double* y, const double* x, long long n, hipStream_t s) { launch_copy_f64(y, x, n, s); }
+  static void rj0(const CsrF64& R, const double* r, const double* ir, double w, double* bc, double* xc, int g,
+                  hipStream_t s) { launch_restrict_j0_f64(R, r, ir, w, bc, xc, g, s); }
   static void pin0(const int* pin, double* b, hipStream_t s) { launch_pin_rhs_f64(pin, b, s); }
   static void dense(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s) {
     launch_dense_solve_f64(n, pin, aug, b, x, s);
@@ -159,6 +161,8 @@ struct Sel<float> {
     launch_spmv_add_f32(A, x, y, g, s);
   }
   static void copy(float* y, const float* x, long long n, hipStream_t s) { launch_copy_f32(y, x, n, s); }
+  static void rj0(const CsrF32& R, const float* r, const float* ir, double w, float* bc, float* xc, int g,
+                  hipStream_t s) { launch_restrict_j0_f32(R, r, ir, (float)w, bc, xc, g, s); }
   static void pin0(const int* pin, float* b, hipStream_t s) { launch_pin_rhs_f32(pin, b, s); }
   static void dense(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s) {
     launch_dense_solve_f32(n, pin, aug, b, x, s);
@@ -361,6 +365,7 @@ void Hierarchy::build_levels(hipStream_t s) {
     } else {
       launch_cf_split(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
     }
+    if (prm_.dominant > 0.0) launch_force_dominant(n, A.rp.p, A.ci.p, A.v.p, prm_.dominant, cf0.p, s);
     if (prm_.anchor >= 0 && prm_.anchor < n) {
       k_set_cf<<<1, 1, 0, s>>>(cf0.p, prm_.anchor);
       IMPES_LAUNCH_CHECK();
@@ -438,7 +443,10 @@ void Hierarchy::build_levels(hipStream_t s) {
       const long long tot = read_scalar(pptr.p + nrows, s);
       IMPES_REQUIRE(tot < (1LL << 40), "amg setup: product list too large");
       pk.alloc(tot);
-      launch_spgemm_expand(nrows, lrp, lci, mrp, mci, pptr.p, pk.p, s);
+      DevBuf<int> ex_e, ex_f;
+      ex_e.alloc(tot);
+      ex_f.alloc(tot);
+      launch_spgemm_expand(nrows, lrp, lci, mrp, mci, pptr.p, pk.p, ex_e.p, ex_f.p, s);
       segmented_sort_u64(pk.p, pptr.p, nrows, max_segment_length(pptr.p, nrows, s), s);
       uc.alloc(nrows);
       launch_spgemm_unique_count(nrows, pptr.p, pk.p, uc.p, s);
@@ -449,7 +457,7 @@ void Hierarchy::build_levels(hipStream_t s) {
       seg.alloc(cnnz + 1);
       pe.alloc(tot);
       pf.alloc(tot);
-      launch_spgemm_finalize(nrows, lrp, lci, mrp, pptr.p, pk.p, crp.p, cci.p, seg.p, pe.p, pf.p, tot, cnnz, s);
+      launch_spgemm_finalize(nrows, pptr.p, pk.p, crp.p, ex_e.p, ex_f.p, cci.p, seg.p, pe.p, pf.p, tot, cnnz, s);
       IMPES_HIP_CHECK(hipStreamSynchronize(s));
     };
     spgemm(n, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf);
@@ -488,12 +496,12 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
   L.dense = L.n <= prm_.dense_max;
   // a single-level hierarchy is solved without the pin: alone it would freeze DOF 0
   L.pin_mode = levels_.size() > 1 ? prm_.coarse_pin : kPinNever;
-  L.pin.alloc(1);
+  L.pin.alloc(2);  // [0] pin flag, [1] inverse produced by the SPD sweep
   launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
   if (!L.dense) return;
   L.aug.alloc(2LL * L.n * L.n);
   launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
-  launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+  launch_gauss_jordan_inverse(L.n, L.aug.p, L.pin.p + 1, s);
 }
 
 void Hierarchy::refresh_f32(hipStream_t s) {
@@ -546,7 +554,7 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
       if (L.dense) {
         launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
-        launch_gauss_jordan_inverse(L.n, L.aug.p, s);
+        launch_gauss_jordan_inverse(L.n, L.aug.p, L.pin.p + 1, s);
       }
     }
   }
@@ -620,7 +628,7 @@ void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
 }
 
 template <class T>
-void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s) {
+void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s, bool first_sweep_done) {
   Level& L = levels_[l];
   if (L.coarsest) {
     coarse_solve<T>(l, b, x, s);
@@ -628,8 +636,8 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
   }
   T* cur = x;
   T* other = Sel<T>::t(L);
-  bool zero = x_zero;
-  smooth<T>(l, b, cur, other, prm_.pre_smooth, zero, s);
+  bool zero = x_zero && !first_sweep_done;
+  smooth<T>(l, b, cur, other, prm_.pre_smooth - (first_sweep_done ? 1 : 0), zero, s);
   T* r = Sel<T>::r(L);
   if (zero) {
     Sel<T>::copy(r, b, L.n, s);  // no pre-smoothing: r = b - A*0
@@ -645,8 +653,14 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
     Sel<T>::res(Sel<T>::A(L), cur, b, r, L.group, s);
   }
   Level& C = levels_[l + 1];
-  Sel<T>::spmv(Sel<T>::R(L), r, Sel<T>::b(C), L.rgroup, s);
-  vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s);
+  if (!C.coarsest && prm_.pre_smooth > 0) {
+    // restriction fused with the coarse level's first sweep from zero
+    Sel<T>::rj0(Sel<T>::R(L), r, Sel<T>::ir(C), prm_.omega, Sel<T>::b(C), Sel<T>::x(C), L.rgroup, s);
+    vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s, true);
+  } else {
+    Sel<T>::spmv(Sel<T>::R(L), r, Sel<T>::b(C), L.rgroup, s);
+    vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s);
+  }
   if (zero) {
     // no pre-smoothing happened: x = P e_c
     Sel<T>::spmv(Sel<T>::P(L), Sel<T>::x(C), cur, L.pgroup, s);

@@ -32,6 +32,9 @@ void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* s
 // whose undecided neighbours become F, until every point is decided (no lift needed)
 void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos,
                               int* deg, signed char* newc, int* undecided, signed char* cf, hipStream_t s);
+// cf[i] = C where |a_ii| > ratio * sum_j!=i |a_ij| (Dirichlet-like rows)
+void launch_force_dominant(int n, const int* rp, const int* ci, const double* v, double ratio, signed char* cf,
+                           hipStream_t s);
 // promote F points with no strong C neighbour
 void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
                     signed char* cf_out, int* is_c, hipStream_t s);
@@ -63,15 +66,16 @@ void launch_transpose_finish(int nnz, const unsigned long long* keys, const doub
 
 // SpGEMM C = L*M, symbolic through expanded products sorted per row
 void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, int* cnt, hipStream_t s);
+// keys[t] = (column, local index); ex_e/ex_f[t] = (left entry, right entry) in expansion order
 void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
-                          const long long* prod_ptr, unsigned long long* keys, hipStream_t s);
+                          const long long* prod_ptr, unsigned long long* keys, int* ex_e, int* ex_f, hipStream_t s);
 void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
                                 hipStream_t s);
 // writes C column indices, product segment pointers (int64, nnzC+1) and the sorted
 // operand index pairs (left entry, right entry) of every product
-void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mrp, const long long* prod_ptr,
-                            const unsigned long long* keys, const int* crp, int* cci, long long* seg, int* pe,
-                            int* pf, long long total_products, int nnz_c, hipStream_t s);
+void launch_spgemm_finalize(int n, const long long* prod_ptr, const unsigned long long* keys, const int* crp,
+                            const int* ex_e, const int* ex_f, int* cci, long long* seg, int* pe, int* pf,
+                            long long total_products, int nnz_c, hipStream_t s);
 // C.v[o] = sum_{t in seg(o)} Lv[pe[t]] * Mv[pf[t]]
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
                            const double* mv, double* cv, hipStream_t s);
@@ -83,7 +87,9 @@ void launch_pin_decide(int n, const int* rp, const int* ci, const double* v, int
 // dense inverse of the (optionally DOF-0-pinned) matrix
 void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, const int* pin, double* dense,
                          hipStream_t s);
-void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s);  // aug is n x 2n row-major, inverse in right half
+// aug is n x 2n row-major [A | I], inverse into the right half: the LDS sweep for symmetric
+// positive definite A (n <= 199), else pivoted Gauss-Jordan; done: device int scratch
+void launch_gauss_jordan_inverse(int n, double* aug, int* done, hipStream_t s);
 // x = Ainv * b (with pin: b[0] treated as 0); FP64 and FP32 vector variants
 void launch_dense_solve_f64(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s);
 void launch_dense_solve_f32(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s);

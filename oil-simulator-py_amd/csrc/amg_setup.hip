@@ -196,6 +196,18 @@ __global__ void k_mis_cover(int n, const int* __restrict__ rp, const int* __rest
   }
 }
 
+__global__ void k_force_dominant(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const double* __restrict__ v, double ratio, signed char* __restrict__ cf) {
+  ROW_LOOP(i, n) {
+    double d = 0.0, off = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      if (ci[e] == (int)i) d += v[e];
+      else off += fabs(v[e]);
+    }
+    if (fabs(d) > ratio * off) cf[i] = kC;
+  }
+}
+
 __global__ void k_orphans(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                           const unsigned char* __restrict__ strong, const signed char* __restrict__ cf_in,
                           signed char* __restrict__ cf_out, int* __restrict__ is_c) {
@@ -484,70 +496,87 @@ __global__ void k_spgemm_count(int n, const int* __restrict__ lrp, const int* __
   }
 }
 
+// One wavefront per output row: lanes cover the right-operand row of each left
+// entry; every product records its key (column, local index) and its operand
+// pair (left entry, right entry) in expansion order.
+#define WAVE_ROW_LOOP(i, n)                                                                  \
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; i < (n); \
+       i += (long long)gridDim.x * blockDim.x / kWave)
+
 __global__ void k_spgemm_expand(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
                                 const int* __restrict__ mrp, const int* __restrict__ mci,
-                                const long long* __restrict__ prod_ptr, unsigned long long* __restrict__ keys) {
-  ROW_LOOP(i, n) {
+                                const long long* __restrict__ prod_ptr, unsigned long long* __restrict__ keys,
+                                int* __restrict__ ex_e, int* __restrict__ ex_f) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
     long long t = prod_ptr[i];
     unsigned local = 0;
     for (int e = lrp[i]; e < lrp[i + 1]; ++e) {
       const int k = lci[e];
-      for (int f = mrp[k]; f < mrp[k + 1]; ++f) {
-        keys[t++] = (((unsigned long long)(unsigned)mci[f]) << 32) | local;
-        ++local;
+      const int f0 = mrp[k], len = mrp[k + 1] - f0;
+      for (int q = lane; q < len; q += kWave) {
+        keys[t + q] = (((unsigned long long)(unsigned)mci[f0 + q]) << 32) | (unsigned long long)(local + q);
+        ex_e[t + q] = e;
+        ex_f[t + q] = f0 + q;
       }
+      t += len;
+      local += (unsigned)len;
     }
   }
 }
 
+// distinct columns per row of the sorted product list (a wavefront per row)
 __global__ void k_spgemm_unique(int n, const long long* __restrict__ prod_ptr,
                                 const unsigned long long* __restrict__ keys, int* __restrict__ cnt) {
-  ROW_LOOP(i, n) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    const long long b = prod_ptr[i], e_end = prod_ptr[i + 1];
     int c = 0;
-    int prev = -1;
-    for (long long t = prod_ptr[i]; t < prod_ptr[i + 1]; ++t) {
-      const int col = (int)(keys[t] >> 32);
-      if (col != prev) { ++c; prev = col; }
+    for (long long t0 = b; t0 < e_end; t0 += kWave) {
+      const long long t = t0 + lane;
+      const bool valid = t < e_end;
+      const int col = valid ? (int)(keys[t] >> 32) : 0;
+      const int prev = (valid && t > b) ? (int)(keys[t - 1] >> 32) : -1;
+      c += __popcll(__ballot(valid && col != prev));
     }
-    cnt[i] = c;
+    if (lane == 0) cnt[i] = c;
   }
 }
 
-// local product index -> (left entry, right entry) by replaying the row's expansion order
-__global__ void k_spgemm_finalize(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
-                                  const int* __restrict__ mrp, const long long* __restrict__ prod_ptr,
+// Output columns and product segments (wave per row: a ballot marks the first
+// product of each column, its rank gives the output slot), and the operand pair
+// of every sorted product, gathered from the expansion-order record.
+__global__ void k_spgemm_finalize(int n, const long long* __restrict__ prod_ptr,
                                   const unsigned long long* __restrict__ keys, const int* __restrict__ crp,
-                                  int* __restrict__ cci, long long* __restrict__ seg, int* __restrict__ pe,
-                                  int* __restrict__ pf, long long total, int nnz_c) {
-  ROW_LOOP(i, n) {
+                                  const int* __restrict__ ex_e, const int* __restrict__ ex_f, int* __restrict__ cci,
+                                  long long* __restrict__ seg, int* __restrict__ pe, int* __restrict__ pf,
+                                  long long total, int nnz_c) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  WAVE_ROW_LOOP(i, n) {
     const long long b = prod_ptr[i], e_end = prod_ptr[i + 1];
-    // the local index l enumerates (e over L row i, f over M row lci[e]) in order;
-    // map l -> (e,f) through the running offsets of L row i.
-    const int le0 = lrp[i], le1 = lrp[i + 1];
     int o = crp[i];
-    int prev = -1;
-    for (long long t = b; t < e_end; ++t) {
-      const unsigned long long key = keys[t];
+    for (long long t0 = b; t0 < e_end; t0 += kWave) {
+      const long long t = t0 + lane;
+      const bool valid = t < e_end;
+      const unsigned long long key = valid ? keys[t] : 0ull;
       const int col = (int)(key >> 32);
-      unsigned l = (unsigned)(key & 0xffffffffull);
-      int e = le0;
-      // locate the left entry whose block of products holds l
-      for (; e < le1; ++e) {
-        const int k = lci[e];
-        const unsigned len = (unsigned)(mrp[k + 1] - mrp[k]);
-        if (l < len) break;
-        l -= len;
+      const int prev = (valid && t > b) ? (int)(keys[t - 1] >> 32) : -1;
+      const bool start = valid && col != prev;
+      const unsigned long long m = __ballot(start);
+      if (valid) {
+        const long long src = b + (long long)(key & 0xffffffffull);
+        pe[t] = ex_e[src];
+        pf[t] = ex_f[src];
       }
-      pe[t] = e;
-      pf[t] = mrp[lci[e]] + (int)l;
-      if (col != prev) {
-        cci[o] = col;
-        seg[o] = t;
-        ++o;
-        prev = col;
+      if (start) {
+        const int pos = o + __popcll(m & below);
+        cci[pos] = col;
+        seg[pos] = t;
       }
+      o += __popcll(m);
     }
-    if (i == n - 1) seg[nnz_c] = total;
+    if (i == n - 1 && lane == 0) seg[nnz_c] = total;
   }
 }
 
@@ -565,6 +594,14 @@ template <class K, class... A>
 void launch_rows(K kernel, long long n, hipStream_t s, A... args) {
   if (n <= 0) return;
   kernel<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(args...);
+  IMPES_LAUNCH_CHECK();
+}
+
+// one wavefront per row
+template <class K, class... A>
+void launch_waves(K kernel, long long n, hipStream_t s, A... args) {
+  if (n <= 0) return;
+  kernel<<<grid_for(n * kWave, kBlock, 1 << 16), kBlock, 0, s>>>(args...);
   IMPES_LAUNCH_CHECK();
 }
 
@@ -607,6 +644,10 @@ void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigne
     if (sum_i32(undecided, n, s) == 0) break;
   }
 }
+void launch_force_dominant(int n, const int* rp, const int* ci, const double* v, double ratio, signed char* cf,
+                           hipStream_t s) {
+  launch_rows(k_force_dominant, n, s, n, rp, ci, v, ratio, cf);
+}
 void launch_orphans(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf_in,
                     signed char* cf_out, int* is_c, hipStream_t s) {
   launch_rows(k_orphans, n, s, n, rp, ci, strong, cf_in, cf_out, is_c);
@@ -646,17 +687,17 @@ void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, 
   launch_rows(k_spgemm_count, n, s, n, lrp, lci, mrp, cnt);
 }
 void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
-                          const long long* prod_ptr, unsigned long long* keys, hipStream_t s) {
-  launch_rows(k_spgemm_expand, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys);
+                          const long long* prod_ptr, unsigned long long* keys, int* ex_e, int* ex_f, hipStream_t s) {
+  launch_waves(k_spgemm_expand, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys, ex_e, ex_f);
 }
 void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
                                 hipStream_t s) {
-  launch_rows(k_spgemm_unique, n, s, n, prod_ptr, keys, cnt);
+  launch_waves(k_spgemm_unique, n, s, n, prod_ptr, keys, cnt);
 }
-void launch_spgemm_finalize(int n, const int* lrp, const int* lci, const int* mrp, const long long* prod_ptr,
-                            const unsigned long long* keys, const int* crp, int* cci, long long* seg, int* pe,
-                            int* pf, long long total_products, int nnz_c, hipStream_t s) {
-  launch_rows(k_spgemm_finalize, n, s, n, lrp, lci, mrp, prod_ptr, keys, crp, cci, seg, pe, pf, total_products, nnz_c);
+void launch_spgemm_finalize(int n, const long long* prod_ptr, const unsigned long long* keys, const int* crp,
+                            const int* ex_e, const int* ex_f, int* cci, long long* seg, int* pe, int* pf,
+                            long long total_products, int nnz_c, hipStream_t s) {
+  launch_waves(k_spgemm_finalize, n, s, n, prod_ptr, keys, crp, ex_e, ex_f, cci, seg, pe, pf, total_products, nnz_c);
 }
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
                            const double* mv, double* cv, hipStream_t s) {

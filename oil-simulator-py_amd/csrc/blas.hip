@@ -205,9 +205,10 @@ void launch_amax_f64(const double* x, long long n, double* out, double* work, hi
 void launch_multidot_f64(const double* V, long long ldv, int k, const double* w, long long n, double* h,
                          double* work, hipStream_t s) {
   IMPES_REQUIRE(k >= 1 && k <= kMaxMultiDot, "multidot: k out of range");
-  k_multidot_partial<<<kRedBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
+  // more waves in flight than the scalar reductions: this one streams k vectors
+  k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
   IMPES_LAUNCH_CHECK();
-  k_fold<false><<<k, kBlock, 0, s>>>(work, kRedBlocks, h);
+  k_fold<false><<<k, kBlock, 0, s>>>(work, kMultiDotBlocks, h);
   IMPES_LAUNCH_CHECK();
 }
 void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h, long long n,

@@ -49,6 +49,7 @@ static AmgParams params_from_c(const impes_amg_params* c) {
     p.trunc = c->trunc;
     p.pmax = c->pmax;
     p.coarsening = c->coarsening;
+    p.dominant = c->dominant;
   }
   IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
   IMPES_REQUIRE(p.pre_smooth >= 0 && p.post_smooth >= 0, "negative sweep count");
@@ -58,6 +59,7 @@ static AmgParams params_from_c(const impes_amg_params* c) {
   IMPES_REQUIRE(p.trunc >= 0.0 && p.trunc < 1.0, "trunc must be in [0, 1)");
   IMPES_REQUIRE(p.pmax >= 0, "pmax must be >= 0");
   IMPES_REQUIRE(p.coarsening == kCoarsenSinglePass || p.coarsening == kCoarsenIterated, "coarsening must be 0 or 1");
+  IMPES_REQUIRE(p.dominant == 0.0 || p.dominant >= 1.0, "dominant must be 0 or >= 1");
   return p;
 }
 
@@ -167,6 +169,7 @@ void impes_amg_default_params(impes_amg_params* p) {
   p->trunc = d.trunc;
   p->pmax = d.pmax;
   p->coarsening = d.coarsening;
+  p->dominant = d.dominant;
 }
 
 int impes_amg_create(const impes_amg_params* params, impes_amg** out) {

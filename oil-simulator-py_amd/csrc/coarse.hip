@@ -60,8 +60,94 @@ __global__ void k_dense_scatter(int n, const int* __restrict__ rp, const int* __
   }
 }
 
-// Single workgroup Gauss-Jordan on the n x 2n augmented matrix (n <= 500).
-__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ aug) {
+// Symmetric positive definite coarsest operators (every Galerkin level of a
+// symmetric M-matrix) are inverted in place in LDS by the sweep operator on the
+// packed lower triangle: for each pivot k, a_ij -= a_ik a_kj / a_kk (i, j != k),
+// a_ik /= a_kk, a_kk = -1/a_kk; after all pivots the triangle holds -A^-1. Without
+// pivoting this is stable for SPD matrices (it is Gauss-Jordan on the symmetric
+// form). One workgroup, the whole triangle in LDS (n <= kSweepMax), two barriers
+// per pivot. A non-symmetric matrix or a non-positive pivot leaves done = 0 and
+// the pivoted Gauss-Jordan below runs instead.
+constexpr int kSweepMax = 199;  // 199*200/2 doubles = 159,200 B of LDS
+constexpr int kSweepPer = (kSweepMax * (kSweepMax + 1) / 2 + 1023) / 1024;
+
+__global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restrict__ aug, int* __restrict__ done) {
+  __shared__ double T[kSweepMax * (kSweepMax + 1) / 2];
+  __shared__ double dg[kSweepMax];  // original diagonal
+  __shared__ int bad;
+  const long long w = 2LL * n;
+  const int tid = threadIdx.x;
+  if (tid == 0) *done = 0;
+  if (n > kSweepMax) return;
+  if (tid == 0) bad = 0;
+  for (int i = tid; i < n; i += blockDim.x) dg[i] = aug[i * w + i];
+  __syncthreads();
+  // symmetric with a positive diagonal? |a_ij - a_ji| <= 1e-10 sqrt(a_ii a_jj): the
+  // rounding asymmetry of a Galerkin product passes, a non-symmetric operator does not
+  // (well rows can exceed interior rows by 15 orders, so the scale is per entry)
+  int local_bad = 0;
+  for (int i = tid; i < n; i += blockDim.x)
+    if (!(dg[i] > 0.0)) local_bad = 1;
+  for (int t = tid; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t % n;
+    if (j >= i) continue;
+    const double d = fabs(aug[i * w + j] - aug[j * w + i]);
+    if (!(d <= 1e-10 * sqrt(fabs(dg[i]) * fabs(dg[j])))) local_bad = 1;
+  }
+  if (local_bad) atomicOr(&bad, 1);
+  __syncthreads();
+  if (bad) return;  // uniform: every thread reads the same LDS flag
+  // packed lower triangle: index i(i+1)/2 + j, j <= i; each thread owns fixed slots
+  const int tot = n * (n + 1) / 2;
+  short oi[kSweepPer], oj[kSweepPer];
+  int cnt = 0;
+  for (int p = tid; p < tot; p += blockDim.x) {
+    int i = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= p) ++i;
+    while (i * (i + 1) / 2 > p) --i;
+    const int j = p - i * (i + 1) / 2;
+    oi[cnt] = (short)i;
+    oj[cnt] = (short)j;
+    ++cnt;
+    T[p] = aug[i * w + j];
+  }
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double d = T[k * (k + 1) / 2 + k];
+    if (!(d > 1e-13 * dg[k])) {  // Schur pivot ~0 relative to a_kk: not (numerically) SPD
+      if (tid == 0) bad = 1;
+      break;
+    }
+    const double inv = 1.0 / d;
+    for (int q = 0; q < cnt; ++q) {
+      const int i = oi[q], j = oj[q];
+      if (i == k || j == k) continue;
+      const double aik = (i > k) ? T[i * (i + 1) / 2 + k] : T[k * (k + 1) / 2 + i];
+      const double ajk = (j > k) ? T[j * (j + 1) / 2 + k] : T[k * (k + 1) / 2 + j];
+      T[i * (i + 1) / 2 + j] -= aik * ajk * inv;
+    }
+    __syncthreads();
+    for (int q = 0; q < cnt; ++q) {
+      const int i = oi[q], j = oj[q];
+      if (i == k && j == k) T[i * (i + 1) / 2 + j] = -inv;
+      else if (i == k || j == k) T[i * (i + 1) / 2 + j] *= inv;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (bad) return;
+  for (int t = tid; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t % n;
+    const int a = i > j ? i : j, b = i > j ? j : i;
+    aug[i * w + n + j] = -T[a * (a + 1) / 2 + b];
+  }
+  if (tid == 0) *done = 1;
+}
+
+// Single workgroup Gauss-Jordan with partial pivoting on the n x 2n augmented
+// matrix (n <= 500); skipped when the sweep already produced the inverse.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ aug, const int* __restrict__ done) {
+  if (*done) return;
   __shared__ double fcol[512];
   __shared__ double red_v[1024 / kWave];
   __shared__ int red_i[1024 / kWave];
@@ -162,9 +248,11 @@ void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, c
   IMPES_LAUNCH_CHECK();
 }
 
-void launch_gauss_jordan_inverse(int n, double* aug, hipStream_t s) {
+void launch_gauss_jordan_inverse(int n, double* aug, int* done, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
-  k_gauss_jordan<<<1, 1024, 0, s>>>(n, aug);
+  k_sweep_inverse<<<1, 1024, 0, s>>>(n, aug, done);
+  IMPES_LAUNCH_CHECK();
+  k_gauss_jordan<<<1, 1024, 0, s>>>(n, aug, done);
   IMPES_LAUNCH_CHECK();
 }
 

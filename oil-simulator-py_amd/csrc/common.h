@@ -18,6 +18,7 @@
 namespace impes {
 
 constexpr int kWave = 64;  // CDNA wavefront width
+constexpr int kXcd = 8;    // accelerator complex dies per MI355X; each has its own L2
 
 // ---------------------------------------------------------------- errors
 struct Error : public std::runtime_error {
@@ -74,6 +75,16 @@ inline int grid_for(long long n, int block, int cap = 8192) {
 }
 
 // ---------------------------------------------------------------- device helpers
+// Workgroups are dispatched round-robin over the XCDs (block b runs on XCD b % 8).
+// This renumbering gives every XCD one contiguous range of logical blocks, so
+// neighbouring rows (stencil planes, the gather footprint of a CSR row range)
+// share one L2 instead of eight.
+__device__ __forceinline__ long long xcd_block() {
+  const unsigned b = blockIdx.x, g = gridDim.x;
+  const unsigned q = g / kXcd, r = g % kXcd, x = b % kXcd;
+  return (long long)x * q + (x < r ? x : r) + b / kXcd;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

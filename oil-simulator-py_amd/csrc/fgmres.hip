@@ -3,6 +3,7 @@
 // one scalar per Arnoldi step to decide termination.
 #include "fgmres.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 #include <cmath>
@@ -102,7 +103,7 @@ void Fgmres::ensure(long long n, int m) {
   bs_.alloc(n);
   Layout L{m};
   small_.alloc(L.size());
-  work_.alloc((size_t)kRedBlocks * (m + 1));
+  work_.alloc((size_t)std::max(kRedBlocks, kMultiDotBlocks) * (m + 1));
 }
 
 void Fgmres::apply(const Operator& A, const double* x, double* y, hipStream_t s) {

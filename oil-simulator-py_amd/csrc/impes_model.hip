@@ -591,15 +591,21 @@ __global__ void k_ledger_fold(int nb, const double* __restrict__ part, double* _
 }
 
 // component-balance integration over the substeps (simulation.py:1396-1503), well cells only
-__global__ void k_substep_ledger(SatWellCtx X, const double* __restrict__ dsw, const double* __restrict__ dsg,
-                                 double* __restrict__ rep) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// Component ledger of the substeps (simulation.py:1396-1503). One thread per
+// (well, substep) pair: it replays the cell's clamped substeps up to its own and
+// adds that substep's surface in/out terms; a fixed-order block reduction keeps the
+// result independent of timing.
+__global__ __launch_bounds__(256) void k_substep_ledger(SatWellCtx X, const double* __restrict__ dsw,
+                                                        const double* __restrict__ dsg, double* __restrict__ rep) {
+  __shared__ double lds[256 / kWave];
   const FluidParams& F = X.F;
   const int N = (int)rep[kNSub];
   const double dt_sub = X.dt / N;
-  double oil_in = rep[kCompOIn], oil_out = rep[kCompOOut], gas_in = rep[kCompGIn], gas_out = rep[kCompGOut];
-  for (int w = 0; w < X.nw; ++w) {
+  double oil_in = 0.0, oil_out = 0.0, gas_in = 0.0, gas_out = 0.0;
+  for (int t = threadIdx.x; t < X.nw * N; t += blockDim.x) {
+    const int w = t / N, sub = t % N;
     const WellDesc& d = X.W[w];
+    if (d.control != 0 && d.control != 1) continue;
     const double* R = rep + kReportHead + kWellSlots * w;
     const int c = d.cell;
     const double P = X.p[c];
@@ -608,59 +614,65 @@ __global__ void k_substep_ledger(SatWellCtx X, const double* __restrict__ dsw, c
     const double rs = pvt_eval(X.T, kRs, P), rv = pvt_eval(X.T, kRv, P);
     const double a = dsw[c] / N, b = dsg[c] / N;
     double s_w = X.sw[c], s_g = X.sg[c];
-    const double q_base = (d.value * F.rate_scale) / kDay;
-    for (int s = 0; s < N; ++s) {
+    for (int q = 0; q <= sub; ++q) {
       double c1, c2;
       substep(F, a, b, s_w, s_g, c1, c2);
-      double kro, krw, krg;
-      relperms(F, s_w, s_g, kro, krw, krg);
-      const double mw = krw / (muw + 1e-12), mo = kro / (muo + 1e-12), mg = krg / (mug + 1e-12);
-      const double mt = mw + mo + mg;
-      const double fw = mw / (mt + 1e-12), fg = mg / (mt + 1e-12), fo = mo / (mt + 1e-12);
-      if (d.control == 0) {
-        if (!d.producer) {
-          if (d.gas) {
-            const double q = d.surface ? q_base * bg : q_base;
-            const double qs = fmax(q, 0.0) / mx12(bg) * kDay;
-            gas_in += qs * (dt_sub / kDay);
-            oil_in += rv * qs * (dt_sub / kDay);
-          }
-        } else {
-          double q;
-          if (d.surface) {
-            if (d.surface_phase == 0) q = -(q_base / mx12(fo / mx12(bo) + fg * rv / mx12(bg)));
-            else if (d.surface_phase == 1) q = -(q_base * bw) / mx12(fw);
-            else if (d.surface_phase == 2) q = -(q_base / mx12(fg / mx12(bg) + fo * rs / mx12(bo)));
-            else q = -(q_base / mx12(fo / mx12(bo) + fg * rv / mx12(bg) + fw / mx12(bw)));
-          } else {
-            q = -q_base;
-          }
-          const double qo_res = fmax(-q * fo, 0.0), qg_res = fmax(-q * fg, 0.0);
-          oil_out += dt_sub * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
-          gas_out += dt_sub * (qg_res / mx12(bg) + rs * qo_res / mx12(bo));
+    }
+    const double q_base = (d.value * F.rate_scale) / kDay;
+    double kro, krw, krg;
+    relperms(F, s_w, s_g, kro, krw, krg);
+    const double mw = krw / (muw + 1e-12), mo = kro / (muo + 1e-12), mg = krg / (mug + 1e-12);
+    const double mt = mw + mo + mg;
+    const double fw = mw / (mt + 1e-12), fg = mg / (mt + 1e-12), fo = mo / (mt + 1e-12);
+    if (d.control == 0) {
+      if (!d.producer) {
+        if (d.gas) {
+          const double q = d.surface ? q_base * bg : q_base;
+          const double qs = fmax(q, 0.0) / mx12(bg) * kDay;
+          gas_in += qs * (dt_sub / kDay);
+          oil_in += rv * qs * (dt_sub / kDay);
         }
-      } else if (d.control == 1) {
-        const double bal = R[kWBal];
-        const double q = (bal == bal) ? -bal * mt / X.mt_pres[c] : d.wi * mt * (P - d.value * 1e6);
-        if (!d.producer) {
-          if (d.gas) {
-            const double qs = fmax(-q, 0.0) / mx12(bg) * kDay;
-            gas_in += qs * (dt_sub / kDay);
-            oil_in += rv * qs * (dt_sub / kDay);
-          }
+      } else {
+        double q;
+        if (d.surface) {
+          if (d.surface_phase == 0) q = -(q_base / mx12(fo / mx12(bo) + fg * rv / mx12(bg)));
+          else if (d.surface_phase == 1) q = -(q_base * bw) / mx12(fw);
+          else if (d.surface_phase == 2) q = -(q_base / mx12(fg / mx12(bg) + fo * rs / mx12(bo)));
+          else q = -(q_base / mx12(fo / mx12(bo) + fg * rv / mx12(bg) + fw / mx12(bw)));
         } else {
-          const double qa = fabs(q);
-          const double qo_res = qa * fo, qg_res = qa * fg;
-          oil_out += dt_sub * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
-          gas_out += dt_sub * (qg_res / mx12(bg) + rs * qo_res / mx12(bo));
+          q = -q_base;
         }
+        const double qo_res = fmax(-q * fo, 0.0), qg_res = fmax(-q * fg, 0.0);
+        oil_out += dt_sub * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
+        gas_out += dt_sub * (qg_res / mx12(bg) + rs * qo_res / mx12(bo));
+      }
+    } else {
+      const double bal = R[kWBal];
+      const double q = (bal == bal) ? -bal * mt / X.mt_pres[c] : d.wi * mt * (P - d.value * 1e6);
+      if (!d.producer) {
+        if (d.gas) {
+          const double qs = fmax(-q, 0.0) / mx12(bg) * kDay;
+          gas_in += qs * (dt_sub / kDay);
+          oil_in += rv * qs * (dt_sub / kDay);
+        }
+      } else {
+        const double qa = fabs(q);
+        const double qo_res = qa * fo, qg_res = qa * fg;
+        oil_out += dt_sub * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
+        gas_out += dt_sub * (qg_res / mx12(bg) + rs * qo_res / mx12(bo));
       }
     }
   }
-  rep[kCompOIn] = oil_in;
-  rep[kCompOOut] = oil_out;
-  rep[kCompGIn] = gas_in;
-  rep[kCompGOut] = gas_out;
+  oil_in = block_sum(oil_in, lds);
+  oil_out = block_sum(oil_out, lds);
+  gas_in = block_sum(gas_in, lds);
+  gas_out = block_sum(gas_out, lds);
+  if (threadIdx.x == 0) {
+    rep[kCompOIn] += oil_in;
+    rep[kCompOOut] += oil_out;
+    rep[kCompGIn] += gas_in;
+    rep[kCompGOut] += gas_out;
+  }
 }
 
 __global__ void k_reset_wells(int nw, const WellDesc* __restrict__ W, double* __restrict__ qw, double* __restrict__ qg,
@@ -855,7 +867,7 @@ void Model::saturation(double dt, const double* p_prev, const double* p_new, con
   k_ledger_fold<<<kLedgerQ, kBlock, 0, s>>>(kRedBlocks, part_.p, report_.p);
   IMPES_LAUNCH_CHECK();
   if (pvt.n > 0 && nw_ > 0) {
-    k_substep_ledger<<<1, 1, 0, s>>>(X, dsw_.p, dsg_.p, report_.p);
+    k_substep_ledger<<<1, 256, 0, s>>>(X, dsw_.p, dsg_.p, report_.p);
     IMPES_LAUNCH_CHECK();
   }
   if (nw_ > 0) {

@@ -9,6 +9,7 @@ namespace impes {
 // ------------------------------------------------------------------ blas.hip
 constexpr int kRedBlocks = 512;  // fixed partial count of two-stage reductions
 constexpr int kMaxMultiDot = 128;
+constexpr int kMultiDotBlocks = 2048;  // partials per vector of launch_multidot_f64 (work: k * 2048 doubles)
 
 void launch_fill_f64(double* x, double v, long long n, hipStream_t s);
 void launch_fill_f32(float* x, float v, long long n, hipStream_t s);
@@ -69,6 +70,12 @@ void launch_jacobi_f64(const CsrF64& A, const double* xi, const double* b, const
 void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const float* inv_relax,
                        float w, float* xo, int group, hipStream_t s);
 // xo = w * inv_relax .* b  (first sweep from a zero guess)
+// bc = R r and xc = w .* inv_relax_c .* bc (restriction fused with the coarse
+// level's first L1-Jacobi sweep from a zero guess)
+void launch_restrict_j0_f64(const CsrF64& R, const double* r, const double* inv_relax_c, double w, double* bc,
+                            double* xc, int group, hipStream_t s);
+void launch_restrict_j0_f32(const CsrF32& R, const float* r, const float* inv_relax_c, float w, float* bc, float* xc,
+                            int group, hipStream_t s);
 void launch_jacobi0_f64(const double* b, const double* inv_relax, double w, double* xo, int n, hipStream_t s);
 void launch_jacobi0_f32(const float* b, const float* inv_relax, float w, float* xo, int n, hipStream_t s);
 

@@ -17,21 +17,24 @@ namespace impes {
 namespace {
 constexpr int kBlock = 256;
 
-enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5 };
+// kRestrictJ0: y = A x (a restriction R r) and y2 = w * aux .* y (the first
+// L1-Jacobi sweep of the coarse level from a zero guess, aux = its inv_relax)
+enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5, kRestrictJ0 = 6 };
 
 template <int G, class T, class TA, int MODE>
 __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                  const T* __restrict__ v, const T* __restrict__ x,
                                                  const T* __restrict__ b, const TA* __restrict__ aux, T w,
-                                                 T* __restrict__ y) {
+                                                 T* __restrict__ y, T* __restrict__ y2) {
   constexpr int RPB = kBlock / G;  // rows per block
   const int g = threadIdx.x % G;
-  for (long long base = (long long)blockIdx.x * RPB; base < n; base += (long long)gridDim.x * RPB) {
+  for (long long base = xcd_block() * RPB; base < n; base += (long long)gridDim.x * RPB) {
     const long long row = base + threadIdx.x / G;
     const bool active = row < n;
     T sum = T(0);
     if (active) {
       const int e1 = rp[row + 1];
+#pragma unroll 4
       for (int e = rp[row] + g; e < e1; e += G) sum += v[e] * x[ci[e]];
     }
     if (G > 1) sum = group_sum<G>(sum);
@@ -42,24 +45,28 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
       else if (MODE == kJac) y[row] = x[row] + w * (T)aux[row] * (b[row] - sum);
       else if (MODE == kRowScaled) y[row] = (T)aux[row] * sum;
       else if (MODE == kRowScaledRes) y[row] = (T)aux[row] * (b[row] - sum);
+      else if (MODE == kRestrictJ0) {
+        y[row] = sum;
+        y2[row] = w * (T)aux[row] * sum;
+      }
     }
   }
 }
 
 template <class T, class TA, int MODE>
 void csr_dispatch(int n, const int* rp, const int* ci, const T* v, const T* x, const T* b, const TA* aux, T w, T* y,
-                  int G, hipStream_t s) {
+                  int G, hipStream_t s, T* y2 = nullptr) {
   if (n <= 0) return;
   const int rpb = kBlock / G;
   const int grid = grid_for(n, rpb, 1 << 18);
   switch (G) {
-    case 1: k_csr<1, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    case 2: k_csr<2, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    case 4: k_csr<4, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    case 8: k_csr<8, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    case 16: k_csr<16, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    case 32: k_csr<32, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
-    default: k_csr<64, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y); break;
+    case 1: k_csr<1, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 2: k_csr<2, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 4: k_csr<4, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 8: k_csr<8, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 16: k_csr<16, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 32: k_csr<32, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    default: k_csr<64, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
   }
   IMPES_LAUNCH_CHECK();
 }
@@ -88,7 +95,7 @@ template <class T, class TA, int MODE>
 __global__ __launch_bounds__(kBlock) void k_st7(Stencil7<T> S, const T* __restrict__ x, const T* __restrict__ b,
                                                 const TA* __restrict__ aux, T w, T* __restrict__ y) {
   const long long n = (long long)S.nx * S.ny * S.nz;
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+  for (long long c = xcd_block() * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
     const int k = (int)(c % S.nz);
     const long long t = c / S.nz;
     const int j = (int)(t % S.ny);
@@ -131,11 +138,14 @@ double st7_bytes(const Stencil7<T>& S, int val_bytes, int vecs) {
   return n * val_bytes * (4 + 1 + vecs);
 }
 
+// Lanes per row: the smallest power of two with at most ~4 entries per lane. A
+// lane then has several independent (value, column, gather) loads in flight,
+// which is what keeps HBM busy: with one entry per lane the sweep is latency-bound.
 int csr_group_for(int n_rows, int nnz) {
   if (n_rows <= 0) return 1;
   const double mean = (double)nnz / (double)n_rows;
   int g = 1;
-  while (g < 64 && g < mean * 0.75) g <<= 1;
+  while (g < 64 && g * 4 < mean) g <<= 1;
   return g;
 }
 
@@ -167,6 +177,14 @@ void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const f
                        int G, hipStream_t s) {
   KScope ks("csr_jacobi_f32", csr_bytes(A.n_rows, A.nnz, 4, 3), s);
   csr_dispatch<float, float, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
+}
+void launch_restrict_j0_f64(const CsrF64& R, const double* r, const double* inv_relax_c, double w, double* bc,
+                            double* xc, int G, hipStream_t s) {
+  csr_dispatch<double, double, kRestrictJ0>(R.n_rows, R.rp, R.ci, R.v, r, nullptr, inv_relax_c, w, bc, G, s, xc);
+}
+void launch_restrict_j0_f32(const CsrF32& R, const float* r, const float* inv_relax_c, float w, float* bc, float* xc,
+                            int G, hipStream_t s) {
+  csr_dispatch<float, float, kRestrictJ0>(R.n_rows, R.rp, R.ci, R.v, r, nullptr, inv_relax_c, w, bc, G, s, xc);
 }
 void launch_jacobi0_f64(const double* b, const double* inv_relax, double w, double* xo, int n, hipStream_t s) {
   if (n <= 0) return;

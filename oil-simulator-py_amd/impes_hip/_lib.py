@@ -40,6 +40,7 @@ class AmgParams(C.Structure):
         ("trunc", C.c_double),
         ("pmax", C.c_int),
         ("coarsening", C.c_int),
+        ("dominant", C.c_double),
     ]
 
 

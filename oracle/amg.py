@@ -27,6 +27,7 @@ how fast the outer solver converges, not the converged answer:
   coarsening  1 = iterated independent-set C/F split (rounds of the same
               local-max selection among undecided points); 0 = the reference's
               single pass with lift, which coarsens poorly on dense deep levels.
+  dominant    rows with |a_ii| > 10 sum|a_ij| (BHP wells) are C points.
 These are pinned by invariants in tests/test_oracle_amg.py (scaling
 invariance, row sums kept, pin decision on Neumann vs. nonsingular operators,
 a maximal independent set checked round by round).
@@ -152,6 +153,23 @@ def cf_split_iterated(A: sp.csr_matrix, strong: np.ndarray):
         cov[r_e[mm]] = True
         und &= ~cov
     return cf, int(np.sum(cf == C_PT))
+
+
+def force_dominant(A: sp.csr_matrix, cf: np.ndarray, ratio: float) -> np.ndarray:
+    """Rows with |a_ii| > ratio * sum_j!=i |a_ij| become C points (not in the
+    reference): a BHP well row is Dirichlet-like, and as an F point its
+    row-normalised interpolation would spread its diagonal over several coarse
+    unknowns as large positive couplings."""
+    if ratio <= 0:
+        return cf
+    rows = _rows(A)
+    off = A.indices != rows
+    offsum = np.bincount(rows[off], weights=np.abs(A.data[off]), minlength=A.shape[0])
+    d = np.zeros(A.shape[0])
+    np.add.at(d, rows[~off], A.data[~off])
+    out = cf.copy()
+    out[np.abs(d) > ratio * offsum] = C_PT
+    return out
 
 
 def promote_orphans(A: sp.csr_matrix, strong: np.ndarray, cf: np.ndarray) -> np.ndarray:
@@ -303,16 +321,16 @@ def inv_relax(A: sp.csr_matrix) -> np.ndarray:
 class Hierarchy:
     """classical_amg.py:699-938 (setup) and :1037-1180 (cycles)."""
 
-    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0}
+    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0, "dominant": 0.0}
 
     def __init__(self, A0: sp.csr_matrix, theta=0.25, max_levels=10, coarsest_size=200, pre=3, post=2,
                  omega=0.7, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, equil_mode=1, coarse_pin=2,
-                 trunc=0.2, pmax=4, coarsening=1):
+                 trunc=0.2, pmax=4, coarsening=1, dominant=10.0):
         self.theta, self.max_levels, self.coarsest_size = theta, max_levels, coarsest_size
         self.pre, self.post, self.omega = pre, post, omega
         self.equil_threshold, self.dense_max, self.coarse_sweeps = equil_threshold, dense_max, coarse_sweeps
         self.equil_mode, self.coarse_pin, self.trunc, self.pmax = equil_mode, coarse_pin, trunc, pmax
-        self.coarsening = coarsening
+        self.coarsening, self.dominant = coarsening, dominant
         A0 = sp.csr_matrix(A0, dtype=np.float64)
         A0.sort_indices()
         self._setup(A0)
@@ -350,6 +368,8 @@ class Hierarchy:
                 break
             s = strength(A, self.theta)
             cf0, ncoarse = (cf_split_iterated if self.coarsening == 1 else cf_split)(A, s)
+            cf0 = force_dominant(A, cf0, self.dominant)
+            ncoarse = int(np.sum(cf0 == C_PT))
             if ncoarse >= 0.9 * n:
                 break
             cf = promote_orphans(A, s, cf0)

@@ -302,7 +302,8 @@ def test_run_writes_the_reference_artifacts(tmp_path):
     z = np.load(os.path.join(out, "case.npz"))
     assert set(z.files) == {"pressure", "sw", "sg", "so"}
     assert np.array_equal(z["pressure"], fl.pressure.double().cpu().numpy())
-    assert sorted(os.listdir(os.path.join(out, "intermediate"))) == ["case_step_2.npz"]
+    # (i + 1) % save_interval == 0 and save_interval < num_steps: steps 2 and 4
+    assert sorted(os.listdir(os.path.join(out, "intermediate"))) == ["case_step_2.npz", "case_step_4.npz"]
     mb = sim.mass_balance
     net = sum(mb[ph]["in"] - mb[ph]["out"] for ph in ("water", "gas", "oil"))
     assert f"total net={net:.6e}" in txt

@@ -126,8 +126,10 @@ def test_hierarchy_matches_oracle(dev, case, mode):
 def test_hierarchy_update_values(dev):
     import impes_hip as H
 
+    # same pattern and well cells (same seed), new transmissibilities: a simulator's
+    # next step (its wells do not move between steps)
     A1, *_ = impes_like(20, 20, 10, seed=5)
-    A2, *_ = impes_like(20, 20, 10, seed=6)  # same pattern, new values
+    A2, *_ = impes_like(20, 20, 10, seed=5, sigma=1.3)
     ho = oa.Hierarchy(A1)
     hg = H.AmgHierarchy()
     hg.setup_csr(*to_dev(A1, dev))
@@ -158,7 +160,7 @@ def test_graph_replay_matches_eager_vcycle(dev, precision):
 
     dims = (24, 20, 6)
     A1, d1, ox1, oy1, oz1 = impes_like(*dims, seed=21)
-    A2, d2, ox2, oy2, oz2 = impes_like(*dims, seed=22)
+    A2, d2, ox2, oy2, oz2 = impes_like(*dims, seed=21, sigma=1.3)  # same well cells
     hg = H.AmgHierarchy()
     hg.setup_st7(dims, t(d1, dev), t(ox1, dev), t(oy1, dev), t(oz1, dev))
     rng = np.random.default_rng(5)

@@ -302,3 +302,38 @@ def test_heterogeneous_nearly_singular_system_converges_in_few_iterations():
     assert sr["iterations"] > 3 * st["iterations"], (st, sr)
     xs = spl.spsolve(A.tocsc(), b)
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_dominant_rows_become_coarse_points():
+    """On an assembled IMPES pressure matrix (the 200k-cell case shrunk to
+    16x16x4, with its BHP producer), the BHP well row (|a_ii| ~ 1e16 x its
+    couplings) is a C point and the coarsest operator stays well conditioned
+    after diagonal scaling. Without the rule the well row is an F point whose
+    row-normalised interpolation spreads its diagonal over several coarse
+    unknowns: the scaled coarsest operator is then singular to ~1e-13."""
+    from oracle import impes as oi
+    from simulator.cases import baseline_case
+
+    cfg, f = baseline_case(1, dims=(16, 16, 4))
+    case = oi.Case(cfg, f, pvt=oi.PVT(cfg["fluid"]["pvt_table"]))
+    ps = oi.pressure_system(case, case.initial_state(), cfg["simulation"]["time_step_days"] * 86400)
+    A = oi.system_csr(case, ps)
+    d = A.diagonal()
+    wells = np.nonzero(d > 1e6 * np.median(d))[0]
+    assert wells.size == 1
+
+    def coarsest_min_eig(H):
+        C = H.levels[-1]["A"].toarray()
+        dg = np.abs(np.diag(C))
+        Cs = C / np.sqrt(np.outer(dg, dg))
+        return np.linalg.eigvalsh((Cs + Cs.T) / 2).min()
+
+    H = oa.Hierarchy(A, coarsest_size=40)
+    assert np.all(H.levels[0]["cf"][wells] == oa.C_PT)
+    assert coarsest_min_eig(H) > 1e-3
+    Hr = oa.Hierarchy(A, coarsest_size=40, dominant=0.0)
+    assert coarsest_min_eig(Hr) < 1e-10
+    b = ps["b"]
+    _, st = oa.fgmres(A, b, precond=H.precondition, rtol=1e-10)
+    _, sr = oa.fgmres(A, b, precond=Hr.precondition, rtol=1e-10)
+    assert st["iterations"] <= sr["iterations"]
