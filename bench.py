@@ -167,7 +167,7 @@ def main():
         bytes_per_launch = kt["bytes"] / kt["launches"]
         achieved = bytes_per_launch / per_launch_s / 1e9
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        tf = os.path.join(ROOT, "profiles", "hbm_traffic_1M_gas_injection.json")
         if os.path.exists(tf):
             traffic = json.load(open(tf)).get(kt["name"], {}).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -101,14 +101,17 @@ __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restric
   const int tot = n * (n + 1) / 2;
   short oi[kSweepPer], oj[kSweepPer];
   int cnt = 0;
-  for (int p = tid; p < tot; p += blockDim.x) {
+#pragma unroll
+  for (int q = 0; q < kSweepPer; ++q) {
+    const int p = tid + q * (int)blockDim.x;
+    if (p >= tot) break;
     int i = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
     while ((i + 1) * (i + 2) / 2 <= p) ++i;
     while (i * (i + 1) / 2 > p) --i;
     const int j = p - i * (i + 1) / 2;
-    oi[cnt] = (short)i;
-    oj[cnt] = (short)j;
-    ++cnt;
+    oi[q] = (short)i;
+    oj[q] = (short)j;
+    cnt = q + 1;
     T[p] = aug[i * w + j];
   }
   __syncthreads();
@@ -119,7 +122,10 @@ __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restric
       break;
     }
     const double inv = 1.0 / d;
-    for (int q = 0; q < cnt; ++q) {
+    // compile-time slot loops: the slot arrays stay in registers with static indices
+#pragma unroll
+    for (int q = 0; q < kSweepPer; ++q) {
+      if (q >= cnt) break;
       const int i = oi[q], j = oj[q];
       if (i == k || j == k) continue;
       const double aik = (i > k) ? T[i * (i + 1) / 2 + k] : T[k * (k + 1) / 2 + i];
@@ -127,7 +133,9 @@ __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restric
       T[i * (i + 1) / 2 + j] -= aik * ajk * inv;
     }
     __syncthreads();
-    for (int q = 0; q < cnt; ++q) {
+#pragma unroll
+    for (int q = 0; q < kSweepPer; ++q) {
+      if (q >= cnt) break;
       const int i = oi[q], j = oj[q];
       if (i == k && j == k) T[i * (i + 1) / 2 + j] = -inv;
       else if (i == k || j == k) T[i * (i + 1) / 2 + j] *= inv;

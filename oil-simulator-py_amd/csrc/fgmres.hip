@@ -81,7 +81,7 @@ __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
 
 Fgmres::Fgmres() {
   const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
-  lookahead_ = !(e && e[0] == '0');
+  lookahead_ = e && e[0] == '1';
 }
 
 Fgmres::~Fgmres() {
