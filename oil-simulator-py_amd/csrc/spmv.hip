@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
     T sum = T(0);
     if (active) {
       const int e1 = rp[row + 1];
-#pragma unroll 6
+#pragma unroll 4
       for (int e = rp[row] + g; e < e1; e += G) sum += v[e] * x[ci[e]];
     }
     if (G > 1) sum = group_sum<G>(sum);
@@ -138,14 +138,14 @@ double st7_bytes(const Stencil7<T>& S, int val_bytes, int vecs) {
   return n * val_bytes * (4 + 1 + vecs);
 }
 
-// Lanes per row: the smallest power of two with at most ~6 entries per lane. A
+// Lanes per row: the smallest power of two with at most ~4 entries per lane. A
 // lane then has several independent (value, column, gather) loads in flight,
 // which is what keeps HBM busy: with one entry per lane the sweep is latency-bound.
 int csr_group_for(int n_rows, int nnz) {
   if (n_rows <= 0) return 1;
   const double mean = (double)nnz / (double)n_rows;
   int g = 1;
-  while (g < 64 && g * 6 < mean) g <<= 1;
+  while (g < 64 && g * 4 < mean) g <<= 1;
   return g;
 }
 
