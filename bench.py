@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--cpu-dims", default="50x50x40")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--kernel", default="st7_jacobi_f32", help="kernel timed for the roofline figure")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing of the history-match case")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (FP64) and exit: the known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters")
@@ -119,6 +120,43 @@ def cpu_baseline(dims, steps):
                       f"{steps} steps after one hierarchy-building step, {el:.1f} s"}
 
 
+def adjoint_timing(steps=None):
+    """The two-phase history-match case (64x64x8, 30 steps of 5 days): wall time of
+    one forward run and of the misfit gradient by the hand-written adjoint
+    (checkpoints every 5 steps, each segment recomputed), both with the HIP AMG in
+    FP64 FGMRES on the GPU, plus the gradient's agreement with a central finite
+    difference along one random unit direction of ln k."""
+    import numpy as np
+    import torch
+    from adjoint import twophase as tp
+    from adjoint.history import problem
+
+    solve = tp.device_solver(rtol=1e-12)
+    P = problem(device="cuda", solve=solve, steps=steps)
+    case, lnk, rates, data, sigma, n = P["case"], P["lnk_prior"], P["rates"], P["data"], P["sigma"], P["steps"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    obs, _ = tp.simulate(case, lnk, rates, solve, n)
+    J0 = float(tp.misfit(obs, data, sigma))
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, g_lnk, g_rates = tp.gradient(case, lnk, rates, data, sigma, solve, n, checkpoint_every=5)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    v = torch.tensor(np.random.default_rng(3).standard_normal(tuple(lnk.shape)), device=lnk.device)
+    v /= v.norm()
+    h = 1e-4
+    jp = float(tp.misfit(tp.simulate(case, lnk + h * v, rates, solve, n)[0], data, sigma))
+    jm = float(tp.misfit(tp.simulate(case, lnk - h * v, rates, solve, n)[0], data, sigma))
+    fd = (jp - jm) / (2 * h)
+    an = float((g_lnk * v).sum())
+    return {"case": f"two-phase history match {'x'.join(map(str, case.dims))}, {n} steps, 5 wells",
+            "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
+            "misfit": J0, "fd_rel_err_one_direction": abs(fd - an) / abs(fd), "grad_lnk_norm": float(g_lnk.norm()),
+            "grad_rates": [float(x) for x in g_rates]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -174,6 +212,9 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kt["name"],
                 "launches": kt["launches"], "avg_us": per_launch_s * 1e6,
                 "algorithmic_bytes_per_launch": bytes_per_launch}
+
+    if not a.no_adjoint:
+        extra["adjoint"] = adjoint_timing()
 
     cpu = None
     if not a.no_cpu:

@@ -34,9 +34,12 @@ __global__ void k_axpy(double* __restrict__ y, double alpha, const double* __res
     y[i] += alpha * x[i];
 }
 
+// `skip` (optional device flag): a nonzero value makes the kernel return at once,
+// so a pass can be switched off on the device without a host round trip
 __global__ void k_dot_partial(const double* __restrict__ x, const double* __restrict__ y, long long n,
-                              double* __restrict__ part) {
+                              double* __restrict__ part, const int* __restrict__ skip) {
   __shared__ double lds[kBlock / kWave];
+  if (skip && *skip) return;
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     s += x[i] * y[i];
@@ -55,8 +58,10 @@ __global__ void k_amax_partial(const double* __restrict__ x, long long n, double
 
 // fold `nb` partials per column for k columns: out[c] = sum_b part[c*nb + b]
 template <bool MAX>
-__global__ void k_fold(const double* __restrict__ part, int nb, double* __restrict__ out) {
+__global__ void k_fold(const double* __restrict__ part, int nb, double* __restrict__ out,
+                       const int* __restrict__ skip = nullptr) {
   __shared__ double lds[kBlock / kWave];
+  if (skip && *skip) return;
   const int c = blockIdx.x;
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -70,8 +75,10 @@ __global__ void k_fold(const double* __restrict__ part, int nb, double* __restri
 // Partial dots of w against k basis vectors. Each thread keeps its w element in
 // a register and streams the k basis entries (coalesced along the row).
 __global__ void k_multidot_partial(const double* __restrict__ V, long long ldv, int k,
-                                   const double* __restrict__ w, long long n, double* __restrict__ part) {
+                                   const double* __restrict__ w, long long n, double* __restrict__ part,
+                                   const int* __restrict__ skip) {
   __shared__ double lds[kBlock / kWave];
+  if (skip && *skip) return;
   // Process basis vectors in chunks of 8 so the per-thread accumulators stay in registers.
   for (int c0 = 0; c0 < k; c0 += 8) {
     const int cn = min(8, k - c0);
@@ -92,7 +99,8 @@ __global__ void k_multidot_partial(const double* __restrict__ V, long long ldv, 
 }
 
 __global__ void k_gs_update(double* __restrict__ w, const double* __restrict__ V, long long ldv, int k,
-                            const double* __restrict__ h, long long n) {
+                            const double* __restrict__ h, long long n, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     double s = w[i];
     for (int c = 0; c < k; ++c) s -= V[(long long)c * ldv + i] * h[c];
@@ -190,10 +198,11 @@ void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipSt
   k_xpby<<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, beta, n);
   IMPES_LAUNCH_CHECK();
 }
-void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s) {
-  k_dot_partial<<<kRedBlocks, kBlock, 0, s>>>(x, y, n, work);
+void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s,
+                    const int* skip) {
+  k_dot_partial<<<kRedBlocks, kBlock, 0, s>>>(x, y, n, work, skip);
   IMPES_LAUNCH_CHECK();
-  k_fold<false><<<1, kBlock, 0, s>>>(work, kRedBlocks, out);
+  k_fold<false><<<1, kBlock, 0, s>>>(work, kRedBlocks, out, skip);
   IMPES_LAUNCH_CHECK();
 }
 void launch_amax_f64(const double* x, long long n, double* out, double* work, hipStream_t s) {
@@ -203,18 +212,18 @@ void launch_amax_f64(const double* x, long long n, double* out, double* work, hi
   IMPES_LAUNCH_CHECK();
 }
 void launch_multidot_f64(const double* V, long long ldv, int k, const double* w, long long n, double* h,
-                         double* work, hipStream_t s) {
+                         double* work, hipStream_t s, const int* skip) {
   IMPES_REQUIRE(k >= 1 && k <= kMaxMultiDot, "multidot: k out of range");
   // more waves in flight than the scalar reductions: this one streams k vectors
-  k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
+  k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work, skip);
   IMPES_LAUNCH_CHECK();
-  k_fold<false><<<k, kBlock, 0, s>>>(work, kMultiDotBlocks, h);
+  k_fold<false><<<k, kBlock, 0, s>>>(work, kMultiDotBlocks, h, skip);
   IMPES_LAUNCH_CHECK();
 }
 void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h, long long n,
-                          hipStream_t s) {
+                          hipStream_t s, const int* skip) {
   if (n <= 0 || k <= 0) return;
-  k_gs_update<<<grid_for(n, kBlock), kBlock, 0, s>>>(w, V, ldv, k, h, n);
+  k_gs_update<<<grid_for(n, kBlock), kBlock, 0, s>>>(w, V, ldv, k, h, n, skip);
   IMPES_LAUNCH_CHECK();
 }
 void launch_lincomb_add_f64(double* x, const double* Z, long long ldz, int k, const double* y, long long n,

@@ -28,15 +28,17 @@ void launch_div_f64(double* y, const double* x, const double* a, long long n, hi
 // y = x + beta * y
 void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipStream_t s);
 // out[0] = sum(x .* y); work >= kRedBlocks doubles
-void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s);
+// skip (optional device flag): nonzero makes the launched kernels return at once
+void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s,
+                    const int* skip = nullptr);
 // out[0] = max |x|
 void launch_amax_f64(const double* x, long long n, double* out, double* work, hipStream_t s);
 // h[i] = V_i . w  for i < k   (V_i = V + i*ldv); work >= kRedBlocks*k doubles
 void launch_multidot_f64(const double* V, long long ldv, int k, const double* w, long long n,
-                         double* h, double* work, hipStream_t s);
+                         double* h, double* work, hipStream_t s, const int* skip = nullptr);
 // w -= sum_i V_i h[i]; if hacc != null also hacc[i] += h[i] (done by block 0)
 void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h,
-                          long long n, hipStream_t s);
+                          long long n, hipStream_t s, const int* skip = nullptr);
 // v_out = w / nrm[0]   (nrm device scalar)
 void launch_scale_by_inv_f64(double* v_out, const double* w, const double* nrm, long long n, hipStream_t s);
 // x += sum_i Z_i y[i]
