@@ -36,16 +36,30 @@ __device__ __forceinline__ bool ranks_above(int deg_a, int a, int deg_b, int b) 
 
 #define ROW_LOOP(i, n) for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
 
+// Eight lanes per row: the row sum of |a_ij| folds with shuffles and the first
+// diagonal entry (lowest position) wins, as in the one-thread-per-row definition.
+constexpr int kDiagLanes = 8;
+
 __global__ void k_diag_relax(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                              const double* __restrict__ v, double* __restrict__ diag, double* __restrict__ ir64,
                              float* __restrict__ ir32) {
-  ROW_LOOP(i, n) {
+  const int g = threadIdx.x % kDiagLanes;
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kDiagLanes; i < n;
+       i += (long long)gridDim.x * blockDim.x / kDiagLanes) {
     double d = 0.0, rs = 0.0;
-    bool found = false;
-    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+    int dpos = 0x7fffffff;
+    const int e1 = rp[i + 1];
+    for (int e = rp[i] + g; e < e1; e += kDiagLanes) {
       rs += fabs(v[e]);
-      if (ci[e] == i && !found) { d = v[e]; found = true; }
+      if (ci[e] == (int)i && e < dpos) { dpos = e; d = v[e]; }
     }
+    rs = group_sum<kDiagLanes>(rs);
+    for (int off = kDiagLanes / 2; off > 0; off >>= 1) {
+      const int op = __shfl_xor(dpos, off, kWave);
+      const double od = __shfl_xor(d, off, kWave);
+      if (op < dpos) { dpos = op; d = od; }
+    }
+    if (g != 0) continue;
     if (diag) diag[i] = d;
     double den = fmax(fabs(d), 0.3 * rs);
     den = fmax(den, 1e-30);
@@ -585,7 +599,9 @@ __global__ void k_spgemm_numeric(int nnz_c, const long long* __restrict__ seg, c
                                  const double* __restrict__ mv, double* __restrict__ cv) {
   ROW_LOOP(o, nnz_c) {
     double s = 0.0;
-    for (long long t = seg[o]; t < seg[o + 1]; ++t) s += lv[pe[t]] * mv[pf[t]];
+    const long long t1 = seg[o + 1];
+#pragma unroll 4
+    for (long long t = seg[o]; t < t1; ++t) s += lv[pe[t]] * mv[pf[t]];
     cv[o] = s;
   }
 }
@@ -609,7 +625,9 @@ void launch_waves(K kernel, long long n, hipStream_t s, A... args) {
 
 void launch_diag_relax(int n, const int* rp, const int* ci, const double* v, double* diag, double* ir64, float* ir32,
                        hipStream_t s) {
-  launch_rows(k_diag_relax, n, s, n, rp, ci, v, diag, ir64, ir32);
+  if (n <= 0) return;
+  k_diag_relax<<<grid_for((long long)n * kDiagLanes, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, v, diag, ir64, ir32);
+  IMPES_LAUNCH_CHECK();
 }
 void launch_count_small_diag(int n, const double* diag, double thr, int* flags, hipStream_t s) {
   launch_rows(k_small_diag, n, s, n, diag, thr, flags);

@@ -73,7 +73,8 @@ constexpr int kSweepPer = (kSweepMax * (kSweepMax + 1) / 2 + 1023) / 1024;
 
 __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restrict__ aug, int* __restrict__ done) {
   __shared__ double T[kSweepMax * (kSweepMax + 1) / 2];
-  __shared__ double dg[kSweepMax];  // original diagonal
+  __shared__ double dg[kSweepMax];    // original diagonal
+  __shared__ double colk[kSweepMax];  // column k of the current pivot, contiguous (conflict-free reads)
   __shared__ int bad;
   const long long w = 2LL * n;
   const int tid = threadIdx.x;
@@ -116,29 +117,24 @@ __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restric
   }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
-    const double d = T[k * (k + 1) / 2 + k];
+    for (int i = tid; i < n; i += blockDim.x) colk[i] = (i >= k) ? T[i * (i + 1) / 2 + k] : T[k * (k + 1) / 2 + i];
+    __syncthreads();
+    const double d = colk[k];
     if (!(d > 1e-13 * dg[k])) {  // Schur pivot ~0 relative to a_kk: not (numerically) SPD
       if (tid == 0) bad = 1;
       break;
     }
     const double inv = 1.0 / d;
-    // compile-time slot loops: the slot arrays stay in registers with static indices
+    // the update reads the staged copy of column k, so the column itself can be
+    // scaled in the same pass; compile-time slot loops keep the slots in registers
 #pragma unroll
     for (int q = 0; q < kSweepPer; ++q) {
       if (q >= cnt) break;
       const int i = oi[q], j = oj[q];
-      if (i == k || j == k) continue;
-      const double aik = (i > k) ? T[i * (i + 1) / 2 + k] : T[k * (k + 1) / 2 + i];
-      const double ajk = (j > k) ? T[j * (j + 1) / 2 + k] : T[k * (k + 1) / 2 + j];
-      T[i * (i + 1) / 2 + j] -= aik * ajk * inv;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kSweepPer; ++q) {
-      if (q >= cnt) break;
-      const int i = oi[q], j = oj[q];
-      if (i == k && j == k) T[i * (i + 1) / 2 + j] = -inv;
-      else if (i == k || j == k) T[i * (i + 1) / 2 + j] *= inv;
+      const int p = i * (i + 1) / 2 + j;
+      if (i == k && j == k) T[p] = -inv;
+      else if (i == k || j == k) T[p] *= inv;
+      else T[p] -= colk[i] * colk[j] * inv;
     }
     __syncthreads();
   }
