@@ -1,6 +1,6 @@
-// Restarted flexible GMRES in FP64 with classical Gram-Schmidt and a selective
-// reorthogonalisation pass, right-preconditioned by the AMG V-cycle
-// (FP64 or FP32 hierarchy). The Krylov method runs on the row-scaled system
+// Restarted flexible GMRES in FP64 with classical Gram-Schmidt and a second,
+// always-run orthogonalisation pass (CGS2), right-preconditioned by the AMG
+// V-cycle (FP64 or FP32 hierarchy). The Krylov method runs on the row-scaled system
 // D^-1 A x = D^-1 b (D = diag A): its residual is a per-cell pressure
 // correction in Pa, so "relative residual" compares like with like even when
 // well rows exceed interior rows by 15 orders of magnitude.
@@ -50,10 +50,6 @@ class Fgmres {
   // for j columns reads). Off by default: on the 1M-cell case the wasted speculative
   // iteration costs more than the wait it hides (56.0M vs 53.9M cell-steps/s).
   bool lookahead_ = false;
-  // Second Gram-Schmidt pass only when the first lost more than half the norm
-  // squared (decided on the device); IMPES_FGMRES_CGS2=1 always runs it (CGS2).
-  bool always_reorth_ = false;
-  DevBuf<int> skip_;
 
  public:
   Fgmres();

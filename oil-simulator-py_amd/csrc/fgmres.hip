@@ -12,6 +12,8 @@ namespace impes {
 
 namespace {
 
+constexpr int kArnoldiBlock = 256;
+
 struct Layout {
   int m;
   __host__ __device__ int H(int i, int j) const { return i + j * (m + 1); }
@@ -35,24 +37,16 @@ __global__ void k_init_cycle(Layout L, double* __restrict__ sm) {
   g[0] = beta;
 }
 
-// After the first classical Gram-Schmidt pass: ||w_before||^2 = ||w||^2 + sum h^2.
-// The second pass runs only if ||w|| < ||w_before|| / sqrt(2) ("twice is enough");
-// otherwise skip = 1 and the second pass's coefficients are zero.
-__global__ void k_reorth_check(Layout L, int j, double* __restrict__ sm, int* __restrict__ skip, int always) {
-  const double* h = sm + L.h();
-  double* h2 = sm + L.h2();
-  const double a = sm[L.sc() + 1];
-  double hh = 0.0;
-  for (int i = 0; i <= j; ++i) hh += h[i] * h[i];
-  const int sk = (!always && a >= 0.5 * (a + hh)) ? 1 : 0;
-  *skip = sk;
-  if (sk)
-    for (int i = 0; i <= j; ++i) h2[i] = 0.0;
-}
-
-// column j of H from the two Gram-Schmidt passes, rotations, residual estimate
-__global__ void k_arnoldi(Layout L, int j, double* __restrict__ sm) {
+// Column j of H from the two Gram-Schmidt passes, rotations, residual estimate.
+// The block first folds the nb partial sums of ||w||^2 left by the second pass.
+__global__ void k_arnoldi(Layout L, int j, double* __restrict__ sm, const double* __restrict__ part, int nb) {
+  __shared__ double lds[kArnoldiBlock / kWave];
+  double ww = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) ww += part[b];
+  ww = block_sum(ww, lds);
+  if (threadIdx.x != 0) return;
   double* sc = sm + L.sc();
+  sc[1] = ww;
   const double* h = sm + L.h();
   const double* h2 = sm + L.h2();
   double* cs = sm + L.cs();
@@ -97,8 +91,6 @@ __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
 Fgmres::Fgmres() {
   const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
   lookahead_ = e && e[0] == '1';
-  const char* r = std::getenv("IMPES_FGMRES_CGS2");
-  always_reorth_ = r && r[0] == '1';
 }
 
 Fgmres::~Fgmres() {
@@ -111,7 +103,6 @@ void Fgmres::ensure(long long n, int m) {
   if (!host_) IMPES_HIP_CHECK(hipHostMalloc((void**)&host_, 8 * sizeof(double), hipHostMallocDefault));
   for (auto& e : est_ev_)
     if (!e) IMPES_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  skip_.alloc(1);
   if (n == n_ && m == m_) return;
   n_ = n;
   m_ = m;
@@ -193,15 +184,12 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
       if (M) M->precondition(precision, vj, zj, s);
       else launch_copy_f64(zj, vj, n, s);
       apply(A, zj, w_.p, s);
+      // CGS2: h = V^T w; w -= V h and h2 = V^T w in one sweep; w -= V h2 with the
+      // partial sums of ||w||^2, folded by k_arnoldi
       launch_multidot_f64(V, n, j + 1, w_.p, n, sm + L.h(), work_.p, s);
-      launch_gs_update_f64(w_.p, V, n, j + 1, sm + L.h(), n, s);
-      launch_dot_f64(w_.p, w_.p, n, sc + 1, work_.p, s);
-      k_reorth_check<<<1, 1, 0, s>>>(L, j, sm, skip_.p, always_reorth_ ? 1 : 0);
-      IMPES_LAUNCH_CHECK();
-      launch_multidot_f64(V, n, j + 1, w_.p, n, sm + L.h2(), work_.p, s, skip_.p);
-      launch_gs_update_f64(w_.p, V, n, j + 1, sm + L.h2(), n, s, skip_.p);
-      launch_dot_f64(w_.p, w_.p, n, sc + 1, work_.p, s, skip_.p);
-      k_arnoldi<<<1, 1, 0, s>>>(L, j, sm);
+      launch_gs_update_multidot_f64(w_.p, V, n, j + 1, sm + L.h(), n, sm + L.h2(), work_.p, s);
+      launch_gs_update_norm_f64(w_.p, V, n, j + 1, sm + L.h2(), n, work_.p, s);
+      k_arnoldi<<<1, kArnoldiBlock, 0, s>>>(L, j, sm, work_.p, kMultiDotBlocks);
       IMPES_LAUNCH_CHECK();
       launch_scale_by_inv_f64(V + (long long)(j + 1) * n, w_.p, sc + 3, n, s);
       post(j);

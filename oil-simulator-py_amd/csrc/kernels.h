@@ -10,6 +10,7 @@ namespace impes {
 constexpr int kRedBlocks = 512;  // fixed partial count of two-stage reductions
 constexpr int kMaxMultiDot = 128;
 constexpr int kMultiDotBlocks = 2048;  // partials per vector of launch_multidot_f64 (work: k * 2048 doubles)
+constexpr int kMaxFusedGs = 32;        // largest k of the register-resident fused Gram-Schmidt kernels
 
 void launch_fill_f64(double* x, double v, long long n, hipStream_t s);
 void launch_fill_f32(float* x, float v, long long n, hipStream_t s);
@@ -28,17 +29,22 @@ void launch_div_f64(double* y, const double* x, const double* a, long long n, hi
 // y = x + beta * y
 void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipStream_t s);
 // out[0] = sum(x .* y); work >= kRedBlocks doubles
-// skip (optional device flag): nonzero makes the launched kernels return at once
-void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s,
-                    const int* skip = nullptr);
+void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s);
 // out[0] = max |x|
 void launch_amax_f64(const double* x, long long n, double* out, double* work, hipStream_t s);
-// h[i] = V_i . w  for i < k   (V_i = V + i*ldv); work >= kRedBlocks*k doubles
+// h[i] = V_i . w  for i < k   (V_i = V + i*ldv); work >= kMultiDotBlocks*k doubles
 void launch_multidot_f64(const double* V, long long ldv, int k, const double* w, long long n,
-                         double* h, double* work, hipStream_t s, const int* skip = nullptr);
-// w -= sum_i V_i h[i]; if hacc != null also hacc[i] += h[i] (done by block 0)
+                         double* h, double* work, hipStream_t s);
+// w -= sum_i V_i h[i]
 void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h,
-                          long long n, hipStream_t s, const int* skip = nullptr);
+                          long long n, hipStream_t s);
+// w -= sum_i V_i h[i], then h2[i] = V_i . w (one sweep over V for k <= kMaxFusedGs);
+// work >= kMultiDotBlocks*k doubles
+void launch_gs_update_multidot_f64(double* w, const double* V, long long ldv, int k, const double* h,
+                                   long long n, double* h2, double* work, hipStream_t s);
+// w -= sum_i V_i h[i], then part[b] (b < kMultiDotBlocks) = partial sums of w . w
+void launch_gs_update_norm_f64(double* w, const double* V, long long ldv, int k, const double* h, long long n,
+                               double* part, hipStream_t s);
 // v_out = w / nrm[0]   (nrm device scalar)
 void launch_scale_by_inv_f64(double* v_out, const double* w, const double* nrm, long long n, hipStream_t s);
 // x += sum_i Z_i y[i]

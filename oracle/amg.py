@@ -466,12 +466,10 @@ class Hierarchy:
         return self.dh * x
 
 
-def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, restart=30, maxit=300,
-           always_reorth=False):
+def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, restart=30, maxit=300):
     """Right-preconditioned restarted FGMRES on D^-1 A x = D^-1 b (same algorithm
-    as the HIP solver): classical Gram-Schmidt with a second pass only when the
-    first leaves less than half of the norm squared (||w||^2 < (||w||^2 + |h|^2)/2,
-    "twice is enough"); always_reorth=True runs CGS2. Returns (x, stats)."""
+    as the HIP solver): classical Gram-Schmidt with a second, always-run pass
+    (CGS2). Returns (x, stats)."""
     A = sp.csr_matrix(A)
     dinv = 1.0 / A.diagonal()
     bs = dinv * b
@@ -499,11 +497,8 @@ def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, re
             w = dinv * (A @ Z[j])
             h = V[:j + 1] @ w
             w = w - V[:j + 1].T @ h
-            a2 = w @ w
-            h2 = np.zeros_like(h)
-            if always_reorth or not a2 >= 0.5 * (a2 + h @ h):
-                h2 = V[:j + 1] @ w
-                w = w - V[:j + 1].T @ h2
+            h2 = V[:j + 1] @ w
+            w = w - V[:j + 1].T @ h2
             H[:j + 1, j] = h + h2
             hn = np.linalg.norm(w)
             H[j + 1, j] = hn
