@@ -115,6 +115,15 @@ class Hierarchy {
   // The V-cycle between the two scalings is replayed from a HIP graph captured on
   // first use after each setup (its buffers are fixed; updates keep them).
   void precondition(int precision, const double* v, double* z, hipStream_t s);
+  // The same preconditioner in three parts, for a caller that fuses the two
+  // scalings into its own kernels: write in_scale() .* v (in the hierarchy's
+  // precision) to vcycle_in(precision), call vcycle_only, then z = out_scale() .*
+  // vcycle_out(precision).
+  const double* in_scale() const { return sin_.p; }
+  const double* out_scale() const { return sout_.p; }
+  void* vcycle_in(int precision);
+  const void* vcycle_out(int precision) const;
+  void vcycle_only(int precision, hipStream_t s);
   void set_graphs(bool on) { graphs_ = on; }
   bool graphs() const { return graphs_; }
 

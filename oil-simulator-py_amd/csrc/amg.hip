@@ -735,23 +735,35 @@ int Hierarchy::cycles(int precision, const double* b, double* x, bool use_x0, in
   return c;
 }
 
-void Hierarchy::precondition(int precision, const double* v, double* z, hipStream_t s) {
+void* Hierarchy::vcycle_in(int precision) {
+  IMPES_REQUIRE(!levels_.empty(), "preconditioner before setup");
+  return precision == 32 ? (void*)levels_[0].b32.p : (void*)levels_[0].b64.p;
+}
+
+const void* Hierarchy::vcycle_out(int precision) const {
+  IMPES_REQUIRE(!levels_.empty(), "preconditioner before setup");
+  return precision == 32 ? (const void*)levels_[0].x32.p : (const void*)levels_[0].x64.p;
+}
+
+void Hierarchy::vcycle_only(int precision, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "preconditioner before setup");
   Level& L0 = levels_[0];
-  const long long n = L0.n;
   const bool replay = graphs_ && !ktimer_enabled();
   const int gi = precision == 32 ? 0 : 1;
   if (replay && !vexec_[gi]) capture_vcycle(precision);
-  if (precision == 32) {
-    launch_scale_f64_to_f32(L0.b32.p, sin_.p, v, n, s);
-    if (replay) IMPES_HIP_CHECK(hipGraphLaunch(vexec_[gi], s));
-    else vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
-    launch_scale_f32_to_f64(z, sout_.p, L0.x32.p, n, s);
-  } else {
-    launch_scale_f64(L0.b64.p, sin_.p, v, n, s);
-    if (replay) IMPES_HIP_CHECK(hipGraphLaunch(vexec_[gi], s));
-    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
-    launch_scale_f64(z, sout_.p, L0.x64.p, n, s);
-  }
+  if (replay) IMPES_HIP_CHECK(hipGraphLaunch(vexec_[gi], s));
+  else if (precision == 32) vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
+  else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
+}
+
+void Hierarchy::precondition(int precision, const double* v, double* z, hipStream_t s) {
+  Level& L0 = levels_[0];
+  const long long n = L0.n;
+  if (precision == 32) launch_scale_f64_to_f32(L0.b32.p, sin_.p, v, n, s);
+  else launch_scale_f64(L0.b64.p, sin_.p, v, n, s);
+  vcycle_only(precision, s);
+  if (precision == 32) launch_scale_f32_to_f64(z, sout_.p, L0.x32.p, n, s);
+  else launch_scale_f64(z, sout_.p, L0.x64.p, n, s);
 }
 
 }  // namespace impes

@@ -100,83 +100,6 @@ __global__ void k_gs_update(double* __restrict__ w, const double* __restrict__ V
   }
 }
 
-// Fused Gram-Schmidt pass of the Arnoldi step for k <= K basis vectors. Each
-// thread loads the k basis entries of its row once into registers, so a single
-// sweep over V both applies one projection and forms the next dots:
-//   UPDATE: w[i] -= sum_c V_c[i] h[c]   (c ascending, as k_gs_update)
-//   DOTS:   part[c * gridDim.x + block] = sum_i V_c[i] w[i]   (w after the update)
-//   NORM:   part[block] = sum_i w[i]^2                           (w after the update)
-// The k wave sums go through LDS in one exchange (one barrier per block).
-template <int K, bool UPDATE, bool DOTS, bool NORM>
-__global__ __launch_bounds__(kBlock) void k_gs_fused(double* __restrict__ w, const double* __restrict__ V,
-                                                     long long ldv, int k, const double* __restrict__ h, long long n,
-                                                     double* __restrict__ part) {
-  constexpr int NC = (DOTS ? K : 0) + (NORM ? 1 : 0);
-  __shared__ double lds[(kBlock / kWave) * NC];
-  double hc[K];
-#pragma unroll
-  for (int t = 0; t < K; ++t) hc[t] = (UPDATE && t < k) ? h[t] : 0.0;
-  double acc[DOTS ? K : 1];
-#pragma unroll
-  for (int t = 0; t < (DOTS ? K : 1); ++t) acc[t] = 0.0;
-  double nrm = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double vv[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) vv[t] = (t < k) ? V[(long long)t * ldv + i] : 0.0;
-    double wi = w[i];
-    if (UPDATE) {
-#pragma unroll
-      for (int t = 0; t < K; ++t)
-        if (t < k) wi -= vv[t] * hc[t];
-      w[i] = wi;
-    }
-    if (DOTS) {
-#pragma unroll
-      for (int t = 0; t < K; ++t) acc[t] += vv[t] * wi;
-    }
-    if (NORM) nrm += wi * wi;
-  }
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  if (DOTS) {
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-      if (t < k) {
-        const double v = wave_sum(acc[t]);
-        if (lane == 0) lds[wid * NC + t] = v;
-      }
-    }
-  }
-  if (NORM) {
-    const double v = wave_sum(nrm);
-    if (lane == 0) lds[wid * NC + NC - 1] = v;
-  }
-  __syncthreads();
-  const int t = threadIdx.x;
-  const bool dot_col = DOTS && t < k, norm_col = NORM && t == NC - 1;
-  if (dot_col || norm_col) {
-    double v = 0.0;
-#pragma unroll
-    for (int q = 0; q < kBlock / kWave; ++q) v += lds[q * NC + t];
-    const int col = dot_col ? t : 0;
-    part[(long long)col * gridDim.x + blockIdx.x] = v;
-  }
-}
-
-template <bool UPDATE, bool DOTS, bool NORM>
-void launch_gs_fused(double* w, const double* V, long long ldv, int k, const double* h, long long n, double* part,
-                     hipStream_t s) {
-  if (k <= 4)
-    k_gs_fused<4, UPDATE, DOTS, NORM><<<kMultiDotBlocks, kBlock, 0, s>>>(w, V, ldv, k, h, n, part);
-  else if (k <= 8)
-    k_gs_fused<8, UPDATE, DOTS, NORM><<<kMultiDotBlocks, kBlock, 0, s>>>(w, V, ldv, k, h, n, part);
-  else if (k <= 16)
-    k_gs_fused<16, UPDATE, DOTS, NORM><<<kMultiDotBlocks, kBlock, 0, s>>>(w, V, ldv, k, h, n, part);
-  else
-    k_gs_fused<kMaxFusedGs, UPDATE, DOTS, NORM><<<kMultiDotBlocks, kBlock, 0, s>>>(w, V, ldv, k, h, n, part);
-  IMPES_LAUNCH_CHECK();
-}
-
 __global__ void k_lincomb_add(double* __restrict__ x, const double* __restrict__ Z, long long ldz, int k,
                               const double* __restrict__ y, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -267,6 +190,10 @@ void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipSt
   k_xpby<<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, beta, n);
   IMPES_LAUNCH_CHECK();
 }
+void launch_dot_partial_f64(const double* x, const double* y, long long n, double* part, int nb, hipStream_t s) {
+  k_dot_partial<<<nb, kBlock, 0, s>>>(x, y, n, part);
+  IMPES_LAUNCH_CHECK();
+}
 void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s) {
   k_dot_partial<<<kRedBlocks, kBlock, 0, s>>>(x, y, n, work);
   IMPES_LAUNCH_CHECK();
@@ -283,12 +210,8 @@ void launch_multidot_f64(const double* V, long long ldv, int k, const double* w,
                          double* work, hipStream_t s) {
   IMPES_REQUIRE(k >= 1 && k <= kMaxMultiDot, "multidot: k out of range");
   // more waves in flight than the scalar reductions: this one streams k vectors
-  if (k <= kMaxFusedGs)
-    launch_gs_fused<false, true, false>(const_cast<double*>(w), V, ldv, k, nullptr, n, work, s);
-  else {
-    k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
-    IMPES_LAUNCH_CHECK();
-  }
+  k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
+  IMPES_LAUNCH_CHECK();
   k_fold<false><<<k, kBlock, 0, s>>>(work, kMultiDotBlocks, h);
   IMPES_LAUNCH_CHECK();
 }
@@ -297,30 +220,6 @@ void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, cons
   if (n <= 0 || k <= 0) return;
   k_gs_update<<<grid_for(n, kBlock), kBlock, 0, s>>>(w, V, ldv, k, h, n);
   IMPES_LAUNCH_CHECK();
-}
-void launch_gs_update_multidot_f64(double* w, const double* V, long long ldv, int k, const double* h,
-                                   long long n, double* h2, double* work, hipStream_t s) {
-  IMPES_REQUIRE(k >= 1 && k <= kMaxMultiDot, "gs_update_multidot: k out of range");
-  if (k <= kMaxFusedGs) {
-    launch_gs_fused<true, true, false>(w, V, ldv, k, h, n, work, s);
-  } else {
-    launch_gs_update_f64(w, V, ldv, k, h, n, s);
-    k_multidot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(V, ldv, k, w, n, work);
-    IMPES_LAUNCH_CHECK();
-  }
-  k_fold<false><<<k, kBlock, 0, s>>>(work, kMultiDotBlocks, h2);
-  IMPES_LAUNCH_CHECK();
-}
-void launch_gs_update_norm_f64(double* w, const double* V, long long ldv, int k, const double* h, long long n,
-                               double* part, hipStream_t s) {
-  IMPES_REQUIRE(k >= 1 && k <= kMaxMultiDot, "gs_update_norm: k out of range");
-  if (k <= kMaxFusedGs) {
-    launch_gs_fused<true, false, true>(w, V, ldv, k, h, n, part, s);
-  } else {
-    launch_gs_update_f64(w, V, ldv, k, h, n, s);
-    k_dot_partial<<<kMultiDotBlocks, kBlock, 0, s>>>(w, w, n, part);
-    IMPES_LAUNCH_CHECK();
-  }
 }
 void launch_lincomb_add_f64(double* x, const double* Z, long long ldz, int k, const double* y, long long n,
                             hipStream_t s) {

@@ -41,7 +41,8 @@ class Fgmres {
   long long n_ = 0;
   int m_ = 0;
   DevBuf<double> V_, Z_, w_, bs_, small_, work_;
-  double* host_ = nullptr;  // pinned scalars: [0] blocking reads, [2..3] per-iteration estimates
+  double* host_ = nullptr;      // pinned scalars: [0] blocking reads, [2..3] per-iteration estimates
+  double* host_dev_ = nullptr;  // device address of host_ (the estimate is stored there by the device)
   hipEvent_t est_ev_[2] = {nullptr, nullptr};
   // Lookahead (IMPES_FGMRES_LOOKAHEAD=1): iteration j is enqueued before the host
   // waits for the residual estimate of iteration j-1, so the GPU never idles on the

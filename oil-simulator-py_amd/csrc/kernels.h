@@ -10,7 +10,6 @@ namespace impes {
 constexpr int kRedBlocks = 512;  // fixed partial count of two-stage reductions
 constexpr int kMaxMultiDot = 128;
 constexpr int kMultiDotBlocks = 2048;  // partials per vector of launch_multidot_f64 (work: k * 2048 doubles)
-constexpr int kMaxFusedGs = 32;        // largest k of the register-resident fused Gram-Schmidt kernels
 
 void launch_fill_f64(double* x, double v, long long n, hipStream_t s);
 void launch_fill_f32(float* x, float v, long long n, hipStream_t s);
@@ -28,6 +27,8 @@ void launch_axpy_f64(double* y, double alpha, const double* x, long long n, hipS
 void launch_div_f64(double* y, const double* x, const double* a, long long n, hipStream_t s);
 // y = x + beta * y
 void launch_xpby_f64(double* y, const double* x, double beta, long long n, hipStream_t s);
+// part[b] (b < nb) = block partial sums of x . y (nb blocks of a grid-stride loop)
+void launch_dot_partial_f64(const double* x, const double* y, long long n, double* part, int nb, hipStream_t s);
 // out[0] = sum(x .* y); work >= kRedBlocks doubles
 void launch_dot_f64(const double* x, const double* y, long long n, double* out, double* work, hipStream_t s);
 // out[0] = max |x|
@@ -38,13 +39,6 @@ void launch_multidot_f64(const double* V, long long ldv, int k, const double* w,
 // w -= sum_i V_i h[i]
 void launch_gs_update_f64(double* w, const double* V, long long ldv, int k, const double* h,
                           long long n, hipStream_t s);
-// w -= sum_i V_i h[i], then h2[i] = V_i . w (one sweep over V for k <= kMaxFusedGs);
-// work >= kMultiDotBlocks*k doubles
-void launch_gs_update_multidot_f64(double* w, const double* V, long long ldv, int k, const double* h,
-                                   long long n, double* h2, double* work, hipStream_t s);
-// w -= sum_i V_i h[i], then part[b] (b < kMultiDotBlocks) = partial sums of w . w
-void launch_gs_update_norm_f64(double* w, const double* V, long long ldv, int k, const double* h, long long n,
-                               double* part, hipStream_t s);
 // v_out = w / nrm[0]   (nrm device scalar)
 void launch_scale_by_inv_f64(double* v_out, const double* w, const double* nrm, long long n, hipStream_t s);
 // x += sum_i Z_i y[i]
@@ -97,6 +91,11 @@ void launch_st7_jacobi_f32(const Stencil7<float>& S, const float* xi, const floa
                            float w, float* xo, hipStream_t s);
 // y = dinv .* (A x)   (row-scaled operator used by the outer Krylov method)
 void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv, const double* x, double* y, hipStream_t s);
+// z = sout .* x (x the FP32 or FP64 V-cycle output) and y = dinv .* (A z) in one pass
+void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,
+                                     const float* x, double* z, double* y, hipStream_t s);
+void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,
+                                     const double* x, double* z, double* y, hipStream_t s);
 void launch_csr_rowscaled_spmv_f64(const CsrF64& A, const double* dinv, const double* x, double* y, int group, hipStream_t s);
 // r = dinv .* (b - A x)
 void launch_st7_rowscaled_residual_f64(const Stencil7<double>& S, const double* dinv, const double* x,

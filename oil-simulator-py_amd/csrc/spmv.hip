@@ -109,6 +109,38 @@ __global__ __launch_bounds__(kBlock) void k_st7(Stencil7<T> S, const T* __restri
   }
 }
 
+// y = dinv .* (A z) and z itself, with z = sout .* x formed on the fly from the
+// V-cycle's output x (FP32 or FP64): the preconditioner's output scaling fused
+// into the Krylov operator apply. Same operation order as k_scale followed by
+// k_st7<kRowScaled>, so the result is identical to the two-kernel form.
+template <class TX>
+__global__ __launch_bounds__(kBlock) void k_st7_rowscaled_z(Stencil7<double> S, const double* __restrict__ dinv,
+                                                            const double* __restrict__ sout,
+                                                            const TX* __restrict__ x, double* __restrict__ z,
+                                                            double* __restrict__ y) {
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  const long long sy = S.nz;
+  const long long sx = (long long)S.ny * S.nz;
+  for (long long c = xcd_block() * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(c % S.nz);
+    const long long t = c / S.nz;
+    const int j = (int)(t % S.ny);
+    const int i = (int)(t / S.ny);
+    auto zv = [&](long long q) { return sout[q] * (double)x[q]; };
+    const double zc = zv(c);
+    double s = 0.0;
+    if (i > 0) s += S.ox[c - sx] * zv(c - sx);
+    if (j > 0) s += S.oy[c - sy] * zv(c - sy);
+    if (k > 0) s += S.oz[c - 1] * zv(c - 1);
+    s += S.d[c] * zc;
+    if (k < S.nz - 1) s += S.oz[c] * zv(c + 1);
+    if (j < S.ny - 1) s += S.oy[c] * zv(c + sy);
+    if (i < S.nx - 1) s += S.ox[c] * zv(c + sx);
+    z[c] = zc;
+    y[c] = dinv[c] * s;
+  }
+}
+
 template <class T>
 __global__ void k_jacobi0(const T* __restrict__ b, const T* __restrict__ inv_relax, T w, T* __restrict__ xo, int n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -229,6 +261,25 @@ void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv
                                    hipStream_t s) {
   KScope ks("st7_rowscaled_spmv_f64", st7_bytes(S, 8, 2), s);
   st7_launch<double, double, kRowScaled>(S, x, nullptr, dinv, 0.0, y, s);
+}
+template <class TX>
+static void st7_rowscaled_z(const Stencil7<double>& S, const double* dinv, const double* sout, const TX* x, double* z,
+                            double* y, hipStream_t s) {
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  if (n <= 0) return;
+  // x and sout, 4 coefficient arrays and dinv read; z and y written
+  KScope ks(sizeof(TX) == 4 ? "st7_rowscaled_z_f32in" : "st7_rowscaled_z_f64in",
+            (double)n * (sizeof(TX) + 8 * 8), s);
+  k_st7_rowscaled_z<TX><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(S, dinv, sout, x, z, y);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,
+                                     const float* x, double* z, double* y, hipStream_t s) {
+  st7_rowscaled_z<float>(S, dinv, sout, x, z, y, s);
+}
+void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,
+                                     const double* x, double* z, double* y, hipStream_t s) {
+  st7_rowscaled_z<double>(S, dinv, sout, x, z, y, s);
 }
 void launch_st7_rowscaled_residual_f64(const Stencil7<double>& S, const double* dinv, const double* x,
                                        const double* b, double* r, hipStream_t s) {
