@@ -74,9 +74,9 @@ typedef struct {
   double theta;          /* strength threshold (reference default 0.25) */
   int max_levels;        /* reference max_levels (hierarchy holds up to max_levels+1 levels) */
   int coarsest_size;     /* stop coarsening at n <= coarsest_size */
-  int pre_smooth;        /* V-cycle pre-sweeps (3) */
+  int pre_smooth;        /* V-cycle pre-sweeps (default 2; reference 3) */
   int post_smooth;       /* V-cycle post-sweeps (2) */
-  double omega;          /* L1-Jacobi damping (0.7) */
+  double omega;          /* L1-Jacobi damping (default 0.8; reference 0.7) */
   double equil_threshold;/* equilibrate when the median |a_ii| < threshold (1e-6) */
   int dense_max;         /* coarsest level solved densely up to this size (500) */
   int coarse_sweeps;     /* Jacobi sweeps on a larger coarsest level (50) */

@@ -2,8 +2,9 @@
 //
 // Mirrors ClassicalAMG of src/solver/classical_amg.py:689-1392: optional
 // symmetric equilibration of the fine matrix, RS coarsening with RS + F-F
-// interpolation, Galerkin coarse operators, L1-Jacobi V(3,2) cycles with
-// omega = 0.7, and a dense DOF-0-pinned coarsest solve. Two value copies of
+// interpolation, Galerkin coarse operators, L1-Jacobi V-cycles (reference:
+// V(3,2), omega = 0.7; default here V(2,2), omega = 0.8), and a dense
+// DOF-0-pinned coarsest solve. Two value copies of
 // every level operator are kept: FP64 and FP32 (the FP32 hierarchy is the
 // preconditioner of the FP64 outer Krylov solver).
 #pragma once
@@ -36,9 +37,11 @@ struct AmgParams {
   double theta = 0.25;
   int max_levels = 10;
   int coarsest_size = 200;
-  int pre_smooth = 3;
+  // L1-Jacobi V-cycle (reference V(3,2), omega 0.7): V(2,2) with omega 0.8 needs as
+  // many outer iterations on the IMPES matrices with one sweep fewer per level
+  int pre_smooth = 2;
   int post_smooth = 2;
-  double omega = 0.7;
+  double omega = 0.8;
   double equil_threshold = 1e-6;  // equilibrate when median |a_ii| is below this
   int dense_max = 500;            // largest coarsest level solved densely
   int coarse_sweeps = 50;         // Jacobi sweeps on a coarsest level above dense_max

@@ -100,9 +100,10 @@ class AmgHierarchy:
     """Owner of an ``impes_amg`` handle (device-resident RS-AMG hierarchy)."""
 
     # Reference behaviour of the four preconditioner choices that are not in the reference.
-    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0, "dominant": 0.0}
+    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0, "dominant": 0.0,
+                 "pre_smooth": 3, "post_smooth": 2, "omega": 0.7}
 
-    def __init__(self, theta=0.25, max_levels=10, coarsest_size=200, pre_smooth=3, post_smooth=2, omega=0.7,
+    def __init__(self, theta=0.25, max_levels=10, coarsest_size=200, pre_smooth=2, post_smooth=2, omega=0.8,
                  equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, anchor=None, equil_mode=1, coarse_pin=2,
                  trunc=0.2, pmax=4, coarsening=1, dominant=10.0):
         p = amg_default_params()

@@ -65,7 +65,8 @@ class Simulator:
         p.max_levels = int(self._amg_cfg.get("max_levels", 10))
         p.coarsest_size = int(self._amg_cfg.get("coarsest_size", 200))
         for key, typ in (("equil_mode", int), ("coarse_pin", int), ("trunc", float), ("pmax", int),
-                         ("coarsening", int), ("dominant", float)):
+                         ("coarsening", int), ("dominant", float), ("pre_smooth", int), ("post_smooth", int),
+                         ("omega", float)):
             if key in self._amg_cfg:
                 setattr(p, key, typ(self._amg_cfg[key]))
         self._model.set_amg(p)

@@ -13,7 +13,7 @@ The reference's energy-minimising blend of interpolation weights
 (classical_amg.py:389-418) indexes the fine-level basis with coarse column
 numbers; it is not restated (documented deviation).
 
-Four preconditioner choices are not in the reference and default to values
+Six preconditioner choices are not in the reference and default to values
 that differ from it (``REFERENCE`` holds the reference behaviour); they change
 how fast the outer solver converges, not the converged answer:
   equil_mode  1 = scale the fine operator by 1/median|a_ii| (frozen at setup)
@@ -28,6 +28,9 @@ how fast the outer solver converges, not the converged answer:
               local-max selection among undecided points); 0 = the reference's
               single pass with lift, which coarsens poorly on dense deep levels.
   dominant    rows with |a_ii| > 10 sum|a_ij| (BHP wells) are C points.
+  pre_smooth/post_smooth/omega
+              V(2,2) L1-Jacobi with omega 0.8 instead of V(3,2) with 0.7: as
+              many outer iterations on the IMPES matrices, one sweep fewer per level.
 These are pinned by invariants in tests/test_oracle_amg.py (scaling
 invariance, row sums kept, pin decision on Neumann vs. nonsingular operators,
 a maximal independent set checked round by round).
@@ -322,13 +325,14 @@ def inv_relax(A: sp.csr_matrix) -> np.ndarray:
 class Hierarchy:
     """classical_amg.py:699-938 (setup) and :1037-1180 (cycles)."""
 
-    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0, "dominant": 0.0}
+    REFERENCE = {"equil_mode": 0, "coarse_pin": 1, "trunc": 0.0, "pmax": 0, "coarsening": 0, "dominant": 0.0,
+                 "pre_smooth": 3, "post_smooth": 2, "omega": 0.7}
 
-    def __init__(self, A0: sp.csr_matrix, theta=0.25, max_levels=10, coarsest_size=200, pre=3, post=2,
-                 omega=0.7, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, equil_mode=1, coarse_pin=2,
+    def __init__(self, A0: sp.csr_matrix, theta=0.25, max_levels=10, coarsest_size=200, pre_smooth=2,
+                 post_smooth=2, omega=0.8, equil_threshold=1e-6, dense_max=500, coarse_sweeps=50, equil_mode=1, coarse_pin=2,
                  trunc=0.2, pmax=4, coarsening=1, dominant=10.0):
         self.theta, self.max_levels, self.coarsest_size = theta, max_levels, coarsest_size
-        self.pre, self.post, self.omega = pre, post, omega
+        self.pre, self.post, self.omega = pre_smooth, post_smooth, omega
         self.equil_threshold, self.dense_max, self.coarse_sweeps = equil_threshold, dense_max, coarse_sweeps
         self.equil_mode, self.coarse_pin, self.trunc, self.pmax = equil_mode, coarse_pin, trunc, pmax
         self.coarsening, self.dominant = coarsening, dominant
