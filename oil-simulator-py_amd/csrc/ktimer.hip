@@ -63,20 +63,37 @@ void ktimer_read(std::string* kernel, double* ms, double* bytes, long long* laun
 
 bool ktimer_enabled() { return st().on; }
 
+namespace {
+thread_local KSlot t_armed;
+thread_local bool t_is_armed = false;
+}  // namespace
+
+KSlot* ktimer_armed() { return t_is_armed ? &t_armed : nullptr; }
+void ktimer_consume() { t_is_armed = false; }
+
 KScope::KScope(const char* kernel, double bytes, hipStream_t s) {
   State& S = st();
   if (!S.on || S.target != kernel) return;
   if (S.used == kPairs) fold();
   slot_ = S.used++;
+  bytes_ = bytes;
   s_ = s;
   S.bytes += bytes;
   S.launches += 1;
-  IMPES_HIP_CHECK(hipEventRecord(S.ev[2 * slot_], s));
+  t_armed = KSlot{S.ev[2 * slot_], S.ev[2 * slot_ + 1]};
+  t_is_armed = true;
 }
 
 KScope::~KScope() {
   if (slot_ < 0) return;
-  (void)hipEventRecord(st().ev[2 * slot_ + 1], s_);
+  if (t_is_armed) {  // nothing was launched (empty operand): an empty interval, counts taken back
+    t_is_armed = false;
+    State& S = st();
+    S.bytes -= bytes_;
+    S.launches -= 1;
+    (void)hipEventRecord(S.ev[2 * slot_], s_);
+    (void)hipEventRecord(S.ev[2 * slot_ + 1], s_);
+  }
 }
 
 }  // namespace impes

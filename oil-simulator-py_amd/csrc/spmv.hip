@@ -60,13 +60,13 @@ void csr_dispatch(int n, const int* rp, const int* ci, const T* v, const T* x, c
   const int rpb = kBlock / G;
   const int grid = grid_for(n, rpb, 1 << 18);
   switch (G) {
-    case 1: k_csr<1, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    case 2: k_csr<2, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    case 4: k_csr<4, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    case 8: k_csr<8, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    case 16: k_csr<16, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    case 32: k_csr<32, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
-    default: k_csr<64, T, TA, MODE><<<grid, kBlock, 0, s>>>(n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 1: klaunch(k_csr<1, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 2: klaunch(k_csr<2, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 4: klaunch(k_csr<4, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 8: klaunch(k_csr<8, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 16: klaunch(k_csr<16, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    case 32: klaunch(k_csr<32, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
+    default: klaunch(k_csr<64, T, TA, MODE>, dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, x, b, aux, w, y, y2); break;
   }
   IMPES_LAUNCH_CHECK();
 }
@@ -151,7 +151,7 @@ template <class T, class TA, int MODE>
 void st7_launch(const Stencil7<T>& S, const T* x, const T* b, const TA* aux, T w, T* y, hipStream_t s) {
   const long long n = (long long)S.nx * S.ny * S.nz;
   if (n <= 0) return;
-  k_st7<T, TA, MODE><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(S, x, b, aux, w, y);
+  klaunch(k_st7<T, TA, MODE>, dim3(grid_for(n, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, x, b, aux, w, y);
   IMPES_LAUNCH_CHECK();
 }
 
@@ -270,7 +270,7 @@ static void st7_rowscaled_z(const Stencil7<double>& S, const double* dinv, const
   // x and sout, 4 coefficient arrays and dinv read; z and y written
   KScope ks(sizeof(TX) == 4 ? "st7_rowscaled_z_f32in" : "st7_rowscaled_z_f64in",
             (double)n * (sizeof(TX) + 8 * 8), s);
-  k_st7_rowscaled_z<TX><<<grid_for(n, kBlock, 1 << 18), kBlock, 0, s>>>(S, dinv, sout, x, z, y);
+  klaunch(k_st7_rowscaled_z<TX>, dim3(grid_for(n, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, dinv, sout, x, z, y);
   IMPES_LAUNCH_CHECK();
 }
 void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,
