@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--kernel", default="st7_jacobi_f32", help="kernel timed for the roofline figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing of the history-match case")
+    ap.add_argument("--no-rl", action="store_true", help="skip the batched reinforcement-learning environment leg")
+    ap.add_argument("--rl-envs", type=int, default=64, help="copies stepped together by the batched environment")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (FP64) and exit: the known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counters")
@@ -157,6 +159,49 @@ def adjoint_timing(steps=None):
             "grad_rates": [float(x) for x in g_rates]}
 
 
+def rl_timing(num_envs, steps=8):
+    """Reinforcement-learning environments (rl_env.py) on the 20x20x3 three-phase
+    environment: env-steps/s of BatchedReservoirEnv with num_envs copies in one
+    device step against ReservoirEnv stepped alone, both with random actions of
+    +-5 % and the observation copied to the host every step (as the reference's
+    step does). Two warmup steps each (the first builds the AMG hierarchy)."""
+    import tempfile
+
+    import numpy as np
+    import torch
+    from rl_env import BatchedReservoirEnv, ReservoirEnv
+    from simulator.cases import rl_env_case
+
+    rng = np.random.default_rng(0)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "env.json")
+        with open(path, "w") as f:
+            json.dump(rl_env_case(), f)
+
+        def timed(env, shape):
+            env.reset(seed=0)
+            for _ in range(2):
+                env.step(rng.uniform(-0.05, 0.05, shape))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                out = env.step(rng.uniform(-0.05, 0.05, shape))
+                if not out[4]["converged"]:
+                    raise RuntimeError("environment step did not converge")
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        benv = BatchedReservoirEnv(path, num_envs)
+        tb = timed(benv, (num_envs, benv.wells_per_env))
+        senv = ReservoirEnv(path)
+        ts = timed(senv, (len(senv.well_manager.wells),))
+    batched = num_envs * steps / tb
+    single = steps / ts
+    return {"case": "three-phase 20x20x3, 4 wells, dt = 1 day, FP32-AMG FGMRES to 1e-8", "num_envs": num_envs,
+            "steps": steps, "batched_ms_per_step": tb / steps * 1e3, "single_ms_per_step": ts / steps * 1e3,
+            "batched_env_steps_per_s": batched, "single_env_steps_per_s": single, "speedup": batched / single}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -215,6 +260,8 @@ def main():
 
     if not a.no_adjoint:
         extra["adjoint"] = adjoint_timing()
+    if not a.no_rl:
+        extra["rl_batch"] = rl_timing(a.rl_envs)
 
     cpu = None
     if not a.no_cpu:

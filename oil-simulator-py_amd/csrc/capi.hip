@@ -461,6 +461,13 @@ int impes_model_set_rock(impes_model* m, const double* kx, const double* ky, con
   });
 }
 
+int impes_model_set_envs(impes_model* m, int env_nx) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m, "null model");
+    m->m.set_envs(env_nx);
+  });
+}
+
 int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(m, "null model");

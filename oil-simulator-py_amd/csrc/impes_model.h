@@ -44,6 +44,11 @@ class Model {
 
   // kx/ky/kz in m^2, pore volume per cell in m^3 (reservoir.py:50)
   void set_rock(const double* kx, const double* ky, const double* kz, const double* pv, hipStream_t s);
+  // The grid holds nx / env_nx independent reservoirs stacked along x (env_nx x-layers
+  // each): no faces between them and a substep count per reservoir (its count in the
+  // report tail). 0 or nx = one reservoir. Call before set_rock.
+  void set_envs(int env_nx);
+  int num_envs() const { return n_env_; }
   void set_wells(const std::vector<WellDesc>& wells);
   void set_amg_params(const AmgParams& p) { amg_prm_ = p; amg_.reset(); }
   void reset_hierarchy() { amg_.reset(); }
@@ -64,7 +69,7 @@ class Model {
 
   long long n() const { return N_; }
   int num_wells() const { return nw_; }
-  int report_len() const { return kReportHead + kWellSlots * nw_; }
+  int report_len() const { return kReportHead + kWellSlots * nw_ + (n_env_ > 1 ? n_env_ : 0); }
   const Hierarchy* hierarchy() const { return amg_.get(); }
 
   Grid grid;
@@ -76,6 +81,9 @@ class Model {
  private:
   long long N_;
   int nw_ = 0;
+  int env_nx_ = 0, n_env_ = 1;
+  long long env_cells_ = 0;          // cells per stacked reservoir
+  DevBuf<double> rat_, nsub_;        // per-cell substep demand, per-reservoir substep count
   bool gravity_ = false;
   DevBuf<double> pvt_P_, pvt_col_, pvt_slope_;
   DevBuf<WellDesc> wells_;

@@ -20,6 +20,7 @@ struct GridDev {
   double dx, dy, dz;
   long long sx, sy;
   int gravity;
+  int env_nx;  // > 0: independent reservoirs of env_nx x-layers each, stacked along x
 };
 
 #define CELL_LOOP(c, n) for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
@@ -41,7 +42,8 @@ __global__ void k_trans(GridDev G, const double* __restrict__ kx, const double* 
     int i, j, k;
     ijk(G, c, i, j, k);
     double t = 0.0;
-    if (i < G.nx - 1) {
+    // no face between two stacked reservoirs (their last and first x-layers)
+    if (i < G.nx - 1 && !(G.env_nx > 0 && (i + 1) % G.env_nx == 0)) {
       const double a = kx[c], b = kx[c + G.sx];
       t = 2 * a * b / (a + b + 1e-15) * G.dy * G.dz / G.dx;
     }
@@ -470,10 +472,11 @@ __device__ __forceinline__ double nmax(double a, double b) {
 }
 
 // simulation.py:1352-1373: saturation increments and the per-cell substep demand
+// (also stored per cell in rat when the grid holds several reservoirs)
 __global__ void k_rat(GridDev G, FluidParams F, double dt, const double* __restrict__ sw, const double* __restrict__ sg,
                       const double* __restrict__ pv, const double* __restrict__ qw, const double* __restrict__ qg,
                       const double* __restrict__ div_w, const double* __restrict__ div_g, double* __restrict__ dsw,
-                      double* __restrict__ dsg, double* __restrict__ part) {
+                      double* __restrict__ dsg, double* __restrict__ part, double* __restrict__ rat) {
   __shared__ double lds[kBlock / kWave];
   const long long n = (long long)G.nx * G.ny * G.nz;
   double m = 0.0;
@@ -490,7 +493,9 @@ __global__ void k_rat(GridDev G, FluidParams F, double dt, const double* __restr
     const double rw = a > 0 ? a / fmax(sw_up, eps) : (-a) / fmax(sw_dn, eps);
     const double rg = g > 0 ? g / fmax(sg_up, eps) : (-g) / fmax(sg_dn, eps);
     const double rl = nmax(fabs(a) / F.max_ds, fabs(g) / F.max_ds);
-    m = nmax(m, nmax(nmax(rw, rg), rl));
+    const double rc = nmax(nmax(rw, rg), rl);
+    if (rat) rat[c] = rc;
+    m = nmax(m, rc);
   }
   // block max with NaN propagation
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -516,6 +521,30 @@ __global__ void k_nsub(int nb, FluidParams F, const double* __restrict__ part, d
   rep[kNSub] = N;
 }
 
+// Substep count of each stacked reservoir (the same rule as k_nsub over its own
+// cells, which are contiguous: env e holds cells [e*cells, (e+1)*cells)). One block
+// per reservoir; nsub[e] also lands in the report tail.
+__global__ void k_nsub_env(long long cells, FluidParams F, const double* __restrict__ rat, double* __restrict__ nsub,
+                           double* __restrict__ rep_tail) {
+  __shared__ double lds[kBlock / kWave];
+  const long long c0 = (long long)blockIdx.x * cells;
+  double m = 0.0;
+  for (long long c = threadIdx.x; c < cells; c += blockDim.x) m = nmax(m, rat[c0 + c]);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int off = kWave / 2; off > 0; off >>= 1) m = nmax(m, __shfl_xor(m, off, kWave));
+  if (lane == 0) lds[wid] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double r = lds[0];
+  for (int q = 1; q < (int)(blockDim.x / kWave); ++q) r = nmax(r, lds[q]);
+  if (!isfinite(r)) r = 0.0;
+  double N = ceil(r);
+  if (N < 1) N = 1;
+  if (N > F.max_substeps) N = F.max_substeps;
+  nsub[blockIdx.x] = N;
+  rep_tail[blockIdx.x] = N;
+}
+
 // one substep of simulation.py:1385-1394 (shared by the field kernel and the ledger kernel)
 __device__ __forceinline__ void substep(const FluidParams& F, double dsw_sub, double dsg_sub, double& s_w, double& s_g,
                                         double& cw, double& cg) {
@@ -535,14 +564,16 @@ __global__ void k_substeps(GridDev G, FluidParams F, PvtDev T, const double* __r
                            const double* __restrict__ p, const double* __restrict__ sw, const double* __restrict__ sg,
                            const double* __restrict__ pv, const double* __restrict__ dsw,
                            const double* __restrict__ dsg, double* __restrict__ sw_out, double* __restrict__ sg_out,
-                           double* __restrict__ so_out, double* __restrict__ part) {
+                           double* __restrict__ so_out, double* __restrict__ part, const double* __restrict__ nsub,
+                           long long env_cells) {
   __shared__ double lds[kBlock / kWave];
   const long long n = (long long)G.nx * G.ny * G.nz;
-  const int N = (int)rep[kNSub];
+  const int N_all = (int)rep[kNSub];
   double acc[kLedgerQ];
 #pragma unroll
   for (int q = 0; q < kLedgerQ; ++q) acc[q] = 0.0;
   CELL_LOOP(c, n) {
+    const int N = nsub ? (int)nsub[c / env_cells] : N_all;  // this cell's reservoir
     const double w0 = sw[c], g0 = sg[c];
     const double a = dsw[c] / N, b = dsg[c] / N;
     double w = w0, g = g0, cw = 0.0, cg = 0.0;
@@ -596,15 +627,20 @@ __global__ void k_ledger_fold(int nb, const double* __restrict__ part, double* _
 // adds that substep's surface in/out terms; a fixed-order block reduction keeps the
 // result independent of timing.
 __global__ __launch_bounds__(256) void k_substep_ledger(SatWellCtx X, const double* __restrict__ dsw,
-                                                        const double* __restrict__ dsg, double* __restrict__ rep) {
+                                                        const double* __restrict__ dsg, double* __restrict__ rep,
+                                                        const double* __restrict__ nsub, long long env_cells) {
   __shared__ double lds[256 / kWave];
   const FluidParams& F = X.F;
-  const int N = (int)rep[kNSub];
-  const double dt_sub = X.dt / N;
+  const int N_all = (int)rep[kNSub];
+  // stacked reservoirs: each well uses its own reservoir's count, at most max_substeps
+  const int Nb = nsub ? (int)F.max_substeps : N_all;
   double oil_in = 0.0, oil_out = 0.0, gas_in = 0.0, gas_out = 0.0;
-  for (int t = threadIdx.x; t < X.nw * N; t += blockDim.x) {
-    const int w = t / N, sub = t % N;
+  for (int t = threadIdx.x; t < X.nw * Nb; t += blockDim.x) {
+    const int w = t / Nb, sub = t % Nb;
     const WellDesc& d = X.W[w];
+    const int N = nsub ? (int)nsub[d.cell / env_cells] : N_all;
+    if (sub >= N) continue;
+    const double dt_sub = X.dt / N;
     if (d.control != 0 && d.control != 1) continue;
     const double* R = rep + kReportHead + kWellSlots * w;
     const int c = d.cell;
@@ -703,13 +739,14 @@ __global__ void k_eval_pvt(PvtDev T, int key, long long n, const double* __restr
   CELL_LOOP(c, n) out[c] = pvt_eval(T, key, p[c]);
 }
 
-GridDev grid_dev(const Grid& g, bool gravity) {
+GridDev grid_dev(const Grid& g, bool gravity, int env_nx) {
   GridDev G;
   G.nx = g.nx; G.ny = g.ny; G.nz = g.nz;
   G.dx = g.dx; G.dy = g.dy; G.dz = g.dz;
   G.sx = (long long)g.ny * g.nz;
   G.sy = g.nz;
   G.gravity = gravity ? 1 : 0;
+  G.env_nx = env_nx;
   return G;
 }
 
@@ -721,6 +758,7 @@ Model::Model(const Grid& g, const FluidParams& f, int pvt_n, const double* pvt_h
   IMPES_REQUIRE(g.nx >= 1 && g.ny >= 1 && g.nz >= 1, "model: bad grid");
   N_ = (long long)g.nx * g.ny * g.nz;
   IMPES_REQUIRE(N_ < (1LL << 30), "model: grid too large for int32 indexing");
+  env_cells_ = N_;
   gravity_ = g.dz > 0 && g.nz > 1;  // simulation.py:999,1594
   if (pvt_n > 0) {
     IMPES_REQUIRE(pvt_n >= 2 && pvt_host, "model: PVT table needs >= 2 rows");
@@ -765,8 +803,26 @@ void Model::set_wells(const std::vector<WellDesc>& w) {
   IMPES_HIP_CHECK(hipMemset(report_.p, 0, sizeof(double) * report_len()));
 }
 
+void Model::set_envs(int env_nx) {
+  IMPES_REQUIRE(env_nx >= 0, "model: env_nx must be >= 0");
+  if (env_nx == 0 || env_nx == grid.nx) {
+    env_nx_ = 0;
+    n_env_ = 1;
+  } else {
+    IMPES_REQUIRE(grid.nx % env_nx == 0, "model: nx must be a multiple of env_nx");
+    env_nx_ = env_nx;
+    n_env_ = grid.nx / env_nx;
+    rat_.alloc(N_);
+    nsub_.alloc(n_env_);
+  }
+  env_cells_ = (long long)(env_nx_ > 0 ? env_nx_ : grid.nx) * grid.ny * grid.nz;
+  amg_.reset();  // the operator's pattern loses the faces between reservoirs
+  report_.alloc(report_len());
+  IMPES_HIP_CHECK(hipMemset(report_.p, 0, sizeof(double) * report_len()));
+}
+
 void Model::set_rock(const double* kx, const double* ky, const double* kz, const double* pv_in, hipStream_t s) {
-  GridDev G = grid_dev(grid, gravity_);
+  GridDev G = grid_dev(grid, gravity_, env_nx_);
   k_trans<<<grid_for(N_, kBlock, 1 << 16), kBlock, 0, s>>>(G, kx, ky, kz, pv_in, tx.p, ty.p, tz.p, pv.p);
   IMPES_LAUNCH_CHECK();
 }
@@ -774,7 +830,7 @@ void Model::set_rock(const double* kx, const double* ky, const double* kz, const
 void Model::assemble(double dt, const double* p, const double* sw, const double* sg, hipStream_t s) {
   IMPES_REQUIRE(dt > 0.0, "assemble: dt must be positive");
   dt_ = dt;
-  GridDev G = grid_dev(grid, gravity_);
+  GridDev G = grid_dev(grid, gravity_, env_nx_);
   const int gr = grid_for(N_, kBlock, 1 << 16);
   k_props_pressure<<<gr, kBlock, 0, s>>>(G, F, pvt, dt, p, sw, sg, pv.p, mob_w_.p, mob_o_.p, mob_g_.p, mob_t.p,
                                          rho_w_.p, rho_o_.p, rho_g_.p, pc_.p, acc.p);
@@ -843,7 +899,7 @@ bool Model::solve_cg(double tol, int maxit, double* x, int& its, hipStream_t s) 
 
 void Model::saturation(double dt, const double* p_prev, const double* p_new, const double* sw, const double* sg,
                        double* sw_out, double* sg_out, double* so_out, double* host_report, hipStream_t s) {
-  GridDev G = grid_dev(grid, gravity_);
+  GridDev G = grid_dev(grid, gravity_, env_nx_);
   const int gr = grid_for(N_, kBlock, 1 << 16);
   k_props_sat<<<gr, kBlock, 0, s>>>(G, F, pvt, p_new, sw, sg, mob_w_.p, mob_o_.p, mob_g_.p, rho_w_.p, rho_g_.p,
                                     pcog_.p);
@@ -856,18 +912,23 @@ void Model::saturation(double dt, const double* p_prev, const double* p_new, con
   // head slots written below must start from zero each step
   k_wells_sat<<<1, 1, 0, s>>>(X, qw_.p, qg_.p, qo_.p, report_.p);
   IMPES_LAUNCH_CHECK();
+  const bool envs = n_env_ > 1;
   k_rat<<<kRedBlocks, kBlock, 0, s>>>(G, F, dt, sw, sg, pv.p, qw_.p, qg_.p, div_w_.p, div_g_.p, dsw_.p, dsg_.p,
-                                      part_.p);
+                                      part_.p, envs ? rat_.p : nullptr);
   IMPES_LAUNCH_CHECK();
   k_nsub<<<1, 1, 0, s>>>(kRedBlocks, F, part_.p, report_.p);
   IMPES_LAUNCH_CHECK();
+  if (envs) {
+    k_nsub_env<<<n_env_, kBlock, 0, s>>>(env_cells_, F, rat_.p, nsub_.p, report_.p + report_len() - n_env_);
+    IMPES_LAUNCH_CHECK();
+  }
   k_substeps<<<kRedBlocks, kBlock, 0, s>>>(G, F, pvt, report_.p, p_new, sw, sg, pv.p, dsw_.p, dsg_.p, sw_out, sg_out,
-                                           so_out, part_.p);
+                                           so_out, part_.p, envs ? nsub_.p : nullptr, env_cells_);
   IMPES_LAUNCH_CHECK();
   k_ledger_fold<<<kLedgerQ, kBlock, 0, s>>>(kRedBlocks, part_.p, report_.p);
   IMPES_LAUNCH_CHECK();
   if (pvt.n > 0 && nw_ > 0) {
-    k_substep_ledger<<<1, 256, 0, s>>>(X, dsw_.p, dsg_.p, report_.p);
+    k_substep_ledger<<<1, 256, 0, s>>>(X, dsw_.p, dsg_.p, report_.p, envs ? nsub_.p : nullptr, env_cells_);
     IMPES_LAUNCH_CHECK();
   }
   if (nw_ > 0) {

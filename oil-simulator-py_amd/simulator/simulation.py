@@ -37,7 +37,10 @@ def _flat(t: torch.Tensor) -> torch.Tensor:
 
 
 class Simulator:
-    def __init__(self, reservoir, fluid, well_manager, sim_params, device=None):
+    def __init__(self, reservoir, fluid, well_manager, sim_params, device=None, *, env_nx=None):
+        """env_nx (not in the reference): the grid holds nx // env_nx independent
+        reservoirs stacked along x (batched environments, rl_env.BatchedReservoirEnv):
+        no flow between them and a substep count each (``last_substeps_env``)."""
         import impes_hip
 
         self.reservoir = reservoir
@@ -56,6 +59,13 @@ class Simulator:
         dx, dy, dz = float(reservoir.dx), float(reservoir.dy), float(reservoir.dz)
         self._model = impes_hip.ImpesModel((nx, ny, nz, dx, dy, dz), fluid.fluid_struct(sim_params),
                                            fluid.pvt_device_table(), self._well_structs())
+        self.num_envs = 1
+        if env_nx:
+            if nx % int(env_nx):
+                raise ValueError(f"nx={nx} is not a multiple of env_nx={env_nx}")
+            self._model.set_envs(int(env_nx))
+            self.num_envs = nx // int(env_nx)
+        self.last_substeps_env = None
         kx, ky, kz = (_flat(t) for t in reservoir.permeability_tensors)
         self.porous_volume = reservoir.porous_volume
         self._model.set_rock(kx, ky, kz, _flat(self.porous_volume))
@@ -198,6 +208,8 @@ class Simulator:
         for i, k in enumerate(("dV_w", "dV_g", "q_w", "q_g", "clamp_w", "clamp_g")):
             vb[k] += rep[12 + i]
         self.last_substeps = int(rep[1])
+        if self.num_envs > 1:
+            self.last_substeps_env = [int(v) for v in rep[len(rep) - self.num_envs:]]
         for w, well in enumerate(self.well_manager.get_wells()):
             R = rep[REPORT_HEAD + WELL_SLOTS * w: REPORT_HEAD + WELL_SLOTS * (w + 1)]
             if R[0] == 0.0:
