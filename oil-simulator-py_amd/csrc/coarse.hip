@@ -148,6 +148,100 @@ __global__ __launch_bounds__(1024) void k_sweep_inverse(int n, double* __restric
   if (tid == 0) *done = 1;
 }
 
+// The same sweep with the matrix in registers (n <= 32 R): the 1024 threads form a
+// 32 x 32 grid and thread (ti, tj) owns rows ti + 32a and columns tj + 32b
+// (a, b < R) of the full symmetric matrix, loaded from the lower triangle so both
+// halves start (and stay) bit-equal. Per pivot only column k passes through LDS,
+// double-buffered so one barrier per pivot suffices; the entry updates are the
+// expressions of k_sweep_inverse in the same order, so the inverse is identical.
+// The packed-LDS form above reads and writes every entry through LDS each pivot.
+template <int R>
+__global__ __launch_bounds__(1024) void k_sweep_regs(int n, double* __restrict__ aug, int* __restrict__ done) {
+  constexpr int S = 32 * R;
+  __shared__ double dg[S];
+  __shared__ double colb[2][S];  // column k of pivot k in colb[k & 1]; zero beyond n
+  __shared__ int bad;
+  const long long w = 2LL * n;
+  const int tid = threadIdx.x;
+  if (tid == 0) *done = 0;
+  if (n > S) return;
+  if (tid == 0) bad = 0;
+  for (int i = tid; i < S; i += blockDim.x) {
+    dg[i] = i < n ? aug[i * w + i] : 1.0;
+    colb[0][i] = 0.0;
+    colb[1][i] = 0.0;
+  }
+  __syncthreads();
+  int local_bad = 0;
+  for (int i = tid; i < n; i += blockDim.x)
+    if (!(dg[i] > 0.0)) local_bad = 1;
+  for (int t = tid; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t % n;
+    if (j >= i) continue;
+    const double d = fabs(aug[i * w + j] - aug[j * w + i]);
+    if (!(d <= 1e-10 * sqrt(fabs(dg[i]) * fabs(dg[j])))) local_bad = 1;
+  }
+  if (local_bad) atomicOr(&bad, 1);
+  __syncthreads();
+  if (bad) return;  // uniform
+  const int ti = tid / 32, tj = tid % 32;
+  double T[R][R];
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      const int hi = i > j ? i : j, lo = i > j ? j : i;
+      T[a][b] = (i < n && j < n) ? aug[hi * w + lo] : 0.0;
+    }
+  for (int k = 0; k < n; ++k) {
+    double* col = colb[k & 1];
+    if (tj == (k & 31)) {
+      const int bk = k >> 5;
+#pragma unroll
+      for (int a = 0; a < R; ++a) {
+        const int i = ti + 32 * a;
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+          if (b == bk && i < n) col[i] = T[a][b];
+      }
+    }
+    __syncthreads();
+    const double d = col[k];
+    if (!(d > 1e-13 * dg[k])) {  // uniform: every thread reads the same LDS value
+      if (tid == 0) bad = 1;
+      break;
+    }
+    const double inv = 1.0 / d;
+    double ci[R], cj[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) ci[a] = col[ti + 32 * a];
+#pragma unroll
+    for (int b = 0; b < R; ++b) cj[b] = col[tj + 32 * b];
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+      const int i = ti + 32 * a;
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        const int j = tj + 32 * b;
+        if (i == k && j == k) T[a][b] = -inv;
+        else if (i == k || j == k) T[a][b] *= inv;
+        else T[a][b] -= ci[a] * cj[b] * inv;
+      }
+    }
+  }
+  __syncthreads();
+  if (bad) return;
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      if (i < n && j < n) aug[i * w + n + j] = -T[a][b];
+    }
+  if (tid == 0) *done = 1;
+}
+
 // Single workgroup Gauss-Jordan with partial pivoting on the n x 2n augmented
 // matrix (n <= 500); skipped when the sweep already produced the inverse.
 __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ aug, const int* __restrict__ done) {
@@ -254,7 +348,12 @@ void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, c
 
 void launch_gauss_jordan_inverse(int n, double* aug, int* done, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
-  k_sweep_inverse<<<1, 1024, 0, s>>>(n, aug, done);
+  // sweep in registers up to 32 x 5 rows (the tile must fit 128 VGPRs of a
+  // 1024-thread block), in packed LDS up to kSweepMax, else Gauss-Jordan only
+  if (n <= 64) k_sweep_regs<2><<<1, 1024, 0, s>>>(n, aug, done);
+  else if (n <= 128) k_sweep_regs<4><<<1, 1024, 0, s>>>(n, aug, done);
+  else if (n <= 160) k_sweep_regs<5><<<1, 1024, 0, s>>>(n, aug, done);
+  else k_sweep_inverse<<<1, 1024, 0, s>>>(n, aug, done);
   IMPES_LAUNCH_CHECK();
   k_gauss_jordan<<<1, 1024, 0, s>>>(n, aug, done);
   IMPES_LAUNCH_CHECK();

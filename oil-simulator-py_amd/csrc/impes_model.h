@@ -33,6 +33,8 @@ enum WellSlot {
   kWellSlots = 12,
 };
 
+constexpr int kWellScratch = 9;  // per-well scratch row of the transport step's well loop
+
 struct Grid {
   int nx, ny, nz;
   double dx, dy, dz;
@@ -87,6 +89,7 @@ class Model {
   bool gravity_ = false;
   DevBuf<double> pvt_P_, pvt_col_, pvt_slope_;
   DevBuf<WellDesc> wells_;
+  DevBuf<double> well_sc_;  // per-well terms of the transport step (kWellScratch doubles per well)
   DevBuf<double> mob_w_, mob_o_, mob_g_, rho_w_, rho_o_, rho_g_, pc_;
   DevBuf<double> pcog_, div_w_, div_g_, dsw_, dsg_, qw_, qg_, qo_;
   DevBuf<double> part_, report_;

@@ -157,19 +157,25 @@ __global__ void k_assemble(GridDev G, FluidParams F, const double* __restrict__ 
 }
 
 // _calculate_well_terms (:1622-1660), applied in well order by one thread.
-__global__ void k_wells_pressure(FluidParams F, int nw, const WellDesc* __restrict__ W, const double* __restrict__ p,
-                                 const double* __restrict__ mt, double* __restrict__ diag, double* __restrict__ b,
-                                 double* __restrict__ dinv, double* __restrict__ rep) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int w = 0; w < nw; ++w) {
+// One thread per well computes its diagonal and right-hand-side increments (into
+// ws[2 w], ws[2 w + 1]) and report slots; one thread then adds the increments in
+// well order (the serial loop's accumulation, bit for bit). One workgroup.
+__global__ __launch_bounds__(256) void k_wells_pressure(FluidParams F, int nw, const WellDesc* __restrict__ W,
+                                                        const double* __restrict__ p, const double* __restrict__ mt,
+                                                        double* __restrict__ diag, double* __restrict__ b,
+                                                        double* __restrict__ dinv, double* __restrict__ rep,
+                                                        double* __restrict__ ws) {
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     double* R = rep + kReportHead + kWellSlots * w;
     const WellDesc& d = W[w];
     const int c = d.cell;
+    double dd = 0.0, db = 0.0;
     R[kWMode] = -1.0;
     R[kWTerm] = 0.0;
     R[kWQRate] = 0.0;
     if (d.control == 0) {
       const double rate = (d.value * F.rate_scale) / kDay * (d.producer ? -1.0 : 1.0);
+      bool bhp = false;
       if (d.producer && d.has_bhp_min) {
         const double denom = d.wi * mt[c] + 1e-12;
         const double preq = p[c] - rate / denom;
@@ -178,20 +184,31 @@ __global__ void k_wells_pressure(FluidParams F, int nw, const WellDesc* __restri
           const double term = d.wi * mt[c];
           R[kWMode] = 1.0;
           R[kWTerm] = term;
-          diag[c] += term;
-          b[c] += term * pmin;
-          continue;
+          dd = term;
+          db = term * pmin;
+          bhp = true;
         }
       }
-      R[kWMode] = 0.0;
-      R[kWQRate] = rate;
-      b[c] += rate;
+      if (!bhp) {
+        R[kWMode] = 0.0;
+        R[kWQRate] = rate;
+        db = rate;
+      }
     } else if (d.control == 1) {
       const double term = d.wi * mt[c];
       R[kWTerm] = term;
-      diag[c] += term;
-      b[c] += term * (d.value * 1e6);
+      dd = term;
+      db = term * (d.value * 1e6);
     }
+    ws[2 * w] = dd;
+    ws[2 * w + 1] = db;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 0; w < nw; ++w) {
+    const int c = W[w].cell;
+    diag[c] += ws[2 * w];
+    b[c] += ws[2 * w + 1];
   }
   for (int w = 0; w < nw; ++w) dinv[W[w].cell] = 1.0 / diag[W[w].cell];
 }
@@ -310,15 +327,25 @@ __device__ double row_balance(const SatWellCtx& X, int c, double q_rate) {
 
 __device__ __forceinline__ double mx12(double a) { return fmax(a, 1e-12); }
 
-// The per-well loop of the transport step (simulation.py:1051-1347), in well order.
-__global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __restrict__ qg, double* __restrict__ qo,
-                            double* __restrict__ rep) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The per-well loop of the transport step (simulation.py:1051-1347). Each well's
+// terms depend only on its cell's state and on the pressure step's report, so one
+// thread per well computes them into its scratch row ws[9 w ..] (cell increments
+// of qw, qg, qo; oil in/out, gas in/out; volume sums of qw, qg); then one thread
+// adds the rows in well order, which gives the sums and the per-cell accumulation
+// of the serial loop bit for bit. One workgroup.
+enum WellScratch { kWsQW = 0, kWsQG, kWsQO, kWsOilIn, kWsOilOut, kWsGasIn, kWsGasOut, kWsSumQW, kWsSumQG,
+                   kWsLen };
+static_assert(kWsLen == kWellScratch, "well scratch row length");
+
+__global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __restrict__ qg,
+                                                   double* __restrict__ qo, double* __restrict__ rep,
+                                                   double* __restrict__ ws) {
   const FluidParams& F = X.F;
   const bool pvt = X.T.n > 0;
   const double dt = X.dt;
-  double oil_in = 0.0, oil_out = 0.0, gas_in = 0.0, gas_out = 0.0, sum_qw = 0.0, sum_qg = 0.0;
-  for (int w = 0; w < X.nw; ++w) {
+  for (int w = threadIdx.x; w < X.nw; w += blockDim.x) {
+    double dqw = 0.0, dqg = 0.0, dqo = 0.0;
+    double oil_in = 0.0, oil_out = 0.0, gas_in = 0.0, gas_out = 0.0, sum_qw = 0.0, sum_qg = 0.0;
     const WellDesc& d = X.W[w];
     double* R = rep + kReportHead + kWellSlots * w;
     R[kWHasQ] = 0.0;
@@ -345,7 +372,7 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
         alpha = (dS > 0.0) ? fmin(alpha, up / mx12(dS)) : fmin(alpha, dn / mx12(fabs(dS)));
         q *= fmax(0.0, fmin(1.0, alpha));
         if (d.gas) {
-          qg[c] += q;
+          dqg += q;
           sum_qg += q;
           if (pvt) {
             const double bg = pvt_eval(X.T, kBg, P), rv = pvt_eval(X.T, kRv, P);
@@ -354,7 +381,7 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
             oil_in += rv * qs * (dt / kDay);
           }
         } else {
-          qw[c] += q;
+          dqw += q;
           sum_qw += q;
         }
       } else {
@@ -401,9 +428,9 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
           oil_out += dt * (fmax(qo_res, 0.0) / mx12(bo) + rv * fmax(qg_res, 0.0) / mx12(bg));
           gas_out += dt * (fmax(qg_res, 0.0) / mx12(bg) + rs * fmax(qo_res, 0.0) / mx12(bo));
         }
-        qw[c] += q * fw;
-        qg[c] += q * fg;
-        qo[c] += q * fo;
+        dqw += q * fw;
+        dqg += q * fg;
+        dqo += q * fo;
         sum_qw += q * fw;
         sum_qg += q * fg;
       }
@@ -428,7 +455,7 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
       }
       if (!d.producer) {
         if (d.gas) {
-          qg[c] += -q;
+          dqg += -q;
           sum_qg += -q;
           if (pvt) {
             const double bg = pvt_eval(X.T, kBg, P), rv = pvt_eval(X.T, kRv, P);
@@ -437,13 +464,13 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
             oil_in += rv * qs * (dt / kDay);
           }
         } else {
-          qw[c] += -q;
+          dqw += -q;
           sum_qw += -q;
         }
       } else {
-        qw[c] += -q * fw;
-        qg[c] += -q * fg;
-        qo[c] += -q * fo;
+        dqw += -q * fw;
+        dqg += -q * fg;
+        dqo += -q * fo;
         sum_qw += -q * fw;
         sum_qg += -q * fg;
         if (pvt) {
@@ -456,6 +483,32 @@ __global__ void k_wells_sat(SatWellCtx X, double* __restrict__ qw, double* __res
         }
       }
     }
+    double* r = ws + (long long)kWsLen * w;
+    r[kWsQW] = dqw;
+    r[kWsQG] = dqg;
+    r[kWsQO] = dqo;
+    r[kWsOilIn] = oil_in;
+    r[kWsOilOut] = oil_out;
+    r[kWsGasIn] = gas_in;
+    r[kWsGasOut] = gas_out;
+    r[kWsSumQW] = sum_qw;
+    r[kWsSumQG] = sum_qg;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double oil_in = 0.0, oil_out = 0.0, gas_in = 0.0, gas_out = 0.0, sum_qw = 0.0, sum_qg = 0.0;
+  for (int w = 0; w < X.nw; ++w) {
+    const double* r = ws + (long long)kWsLen * w;
+    const int c = X.W[w].cell;
+    qw[c] += r[kWsQW];
+    qg[c] += r[kWsQG];
+    qo[c] += r[kWsQO];
+    oil_in += r[kWsOilIn];
+    oil_out += r[kWsOilOut];
+    gas_in += r[kWsGasIn];
+    gas_out += r[kWsGasOut];
+    sum_qw += r[kWsSumQW];
+    sum_qg += r[kWsSumQG];
   }
   rep[kCompOIn] = oil_in;
   rep[kCompOOut] = oil_out;
@@ -509,10 +562,23 @@ __global__ void k_rat(GridDev G, FluidParams F, double dt, const double* __restr
   }
 }
 
+// NaN-propagating max of the block's values (order-independent); valid on thread 0
+__device__ double block_nmax(double m, double* lds) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  for (int off = kWave / 2; off > 0; off >>= 1) m = nmax(m, __shfl_xor(m, off, kWave));
+  if (lane == 0) lds[wid] = m;
+  __syncthreads();
+  double r = lds[0];
+  for (int q = 1; q < (int)(blockDim.x / kWave); ++q) r = nmax(r, lds[q]);
+  return r;
+}
+
 __global__ void k_nsub(int nb, FluidParams F, const double* __restrict__ part, double* __restrict__ rep) {
+  __shared__ double lds[kBlock / kWave];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < nb; q += blockDim.x) m = nmax(m, part[q]);
+  double r = block_nmax(m, lds);
   if (threadIdx.x != 0) return;
-  double r = 0.0;
-  for (int q = 0; q < nb; ++q) r = nmax(r, part[q]);
   if (!isfinite(r)) r = 0.0;  // nan_to_num(nan=0, posinf=0, neginf=0)
   double N = ceil(r);
   if (N < 1) N = 1;
@@ -530,13 +596,8 @@ __global__ void k_nsub_env(long long cells, FluidParams F, const double* __restr
   const long long c0 = (long long)blockIdx.x * cells;
   double m = 0.0;
   for (long long c = threadIdx.x; c < cells; c += blockDim.x) m = nmax(m, rat[c0 + c]);
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  for (int off = kWave / 2; off > 0; off >>= 1) m = nmax(m, __shfl_xor(m, off, kWave));
-  if (lane == 0) lds[wid] = m;
-  __syncthreads();
+  double r = block_nmax(m, lds);
   if (threadIdx.x != 0) return;
-  double r = lds[0];
-  for (int q = 1; q < (int)(blockDim.x / kWave); ++q) r = nmax(r, lds[q]);
   if (!isfinite(r)) r = 0.0;
   double N = ceil(r);
   if (N < 1) N = 1;
@@ -797,6 +858,7 @@ void Model::set_wells(const std::vector<WellDesc>& w) {
   nw_ = (int)w.size();
   if (nw_ > 0) {
     wells_.alloc(nw_);
+    well_sc_.alloc((size_t)kWellScratch * nw_);
     IMPES_HIP_CHECK(hipMemcpy(wells_.p, w.data(), sizeof(WellDesc) * nw_, hipMemcpyHostToDevice));
   }
   report_.alloc(report_len());
@@ -839,7 +901,7 @@ void Model::assemble(double dt, const double* p, const double* sw, const double*
                                    rho_o_.p, rho_g_.p, pc_.p, acc.p, diag.p, ox.p, oy.p, oz.p, b.p, extra.p, dinv.p);
   IMPES_LAUNCH_CHECK();
   if (nw_ > 0) {
-    k_wells_pressure<<<1, 1, 0, s>>>(F, nw_, wells_.p, p, mob_t.p, diag.p, b.p, dinv.p, report_.p);
+    k_wells_pressure<<<1, 256, 0, s>>>(F, nw_, wells_.p, p, mob_t.p, diag.p, b.p, dinv.p, report_.p, well_sc_.p);
     IMPES_LAUNCH_CHECK();
   }
 }
@@ -910,13 +972,13 @@ void Model::saturation(double dt, const double* p_prev, const double* p_new, con
   SatWellCtx X{G, F, pvt, dt, nw_, wells_.p, p_prev, p_new, sw, sg, pv.p, mob_w_.p, mob_o_.p, mob_g_.p, mob_t.p,
                ox.p, oy.p, oz.p, acc.p, extra.p};
   // head slots written below must start from zero each step
-  k_wells_sat<<<1, 1, 0, s>>>(X, qw_.p, qg_.p, qo_.p, report_.p);
+  k_wells_sat<<<1, 256, 0, s>>>(X, qw_.p, qg_.p, qo_.p, report_.p, well_sc_.p);
   IMPES_LAUNCH_CHECK();
   const bool envs = n_env_ > 1;
   k_rat<<<kRedBlocks, kBlock, 0, s>>>(G, F, dt, sw, sg, pv.p, qw_.p, qg_.p, div_w_.p, div_g_.p, dsw_.p, dsg_.p,
                                       part_.p, envs ? rat_.p : nullptr);
   IMPES_LAUNCH_CHECK();
-  k_nsub<<<1, 1, 0, s>>>(kRedBlocks, F, part_.p, report_.p);
+  k_nsub<<<1, kBlock, 0, s>>>(kRedBlocks, F, part_.p, report_.p);
   IMPES_LAUNCH_CHECK();
   if (envs) {
     k_nsub_env<<<n_env_, kBlock, 0, s>>>(env_cells_, F, rat_.p, nsub_.p, report_.p + report_len() - n_env_);
