@@ -159,6 +159,55 @@ def adjoint_timing(steps=None):
             "grad_rates": [float(x) for x in g_rates]}
 
 
+def adjoint3_timing(steps=20, checkpoint_every=5):
+    """The adjoint of the benchmark case itself (1M cells, three-phase gas
+    injection, 20 steps of 1 day): wall time of one forward run and of the gradient
+    of the final oil in place (surface m3) with respect to per-cell ln k and the
+    well controls (torch reverse sweep, checkpoints every 5 steps with
+    recomputation), both with FP32 RS-AMG inside FP64 FGMRES to 1e-10 for the
+    forward and the transposed solves; plus a central finite difference of the
+    objective along one random unit direction of ln k (h = 1e-4)."""
+    import numpy as np
+    import torch
+    from adjoint import threephase as tp
+    from adjoint.twophase import device_solver
+    from simulator.cases import baseline_case
+
+    cfg, fields = baseline_case(2)
+    case = tp.ThreePhaseCase.from_config(cfg, fields, device="cuda")
+    solve = device_solver(rtol=1e-10, precision=32)
+
+    def J_of(lnk):
+        with torch.no_grad():
+            p, sw, sg, counts = tp.simulate(case, lnk, case.rates0, solve, steps)
+            return tp.oil_in_place(case, p, sw, sg).item(), counts
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J0, counts = J_of(case.lnk0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    J, g_lnk, g_val, counts_g = tp.gradient(case, case.lnk0, case.rates0, solve, steps, checkpoint_every)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    v = torch.tensor(np.random.default_rng(3).standard_normal(case.dims), device="cuda")
+    v /= v.norm()
+    h = 1e-4
+    (jp, cp), (jm, cm) = J_of(case.lnk0 + h * v), J_of(case.lnk0 - h * v)
+    fd = (jp - jm) / (2 * h)
+    an = float((g_lnk * v).sum())
+    return {"case": f"three-phase gas injection {'x'.join(map(str, case.dims))}, {steps} steps, "
+                    f"{len(case.wells)} wells; objective: final oil in place (surface m3)",
+            "forward_s": t_fwd, "gradient_s": t_grad, "gradient_over_forward": t_grad / t_fwd,
+            "peak_gb": peak / 1e9, "substeps": counts, "objective": J0,
+            "fd_rel_err_one_direction": abs(fd - an) / max(abs(fd), 1e-300),
+            "fd_substeps_unchanged": cp == cm == counts, "grad_lnk_norm": float(g_lnk.norm()),
+            "grad_well_controls": [float(x) for x in g_val]}
+
+
 def rl_timing(num_envs, steps=8):
     """Reinforcement-learning environments (rl_env.py) on the 20x20x3 three-phase
     environment: env-steps/s of BatchedReservoirEnv with num_envs copies in one
@@ -260,6 +309,7 @@ def main():
 
     if not a.no_adjoint:
         extra["adjoint"] = adjoint_timing()
+        extra["adjoint_three_phase"] = adjoint3_timing()
     if not a.no_rl:
         extra["rl_batch"] = rl_timing(a.rl_envs)
 

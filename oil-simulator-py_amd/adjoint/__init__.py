@@ -1,4 +1,7 @@
-"""Hand-written discrete adjoint of the IMPES step (two-phase oil-water)."""
+"""Discrete adjoints of the IMPES step: hand-written reverse sweep for two-phase
+oil-water cases (``twophase``) and the three-phase black-oil step with torch
+autograd around the HIP pressure solve (``threephase``)."""
+from . import threephase  # noqa: F401
 from .twophase import (  # noqa: F401
     HistoryMatchMisfit,
     TwoPhaseCase,
