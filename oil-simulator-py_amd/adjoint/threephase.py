@@ -223,6 +223,23 @@ class ThreePhaseCase:
                 torch.full(self.dims, f["sg0"], **f64))
 
 
+# ---------------------------------------------------------------- branch record
+def _record(branches, mask):
+    """Appends (count, sum of positions) of a boolean decision mask: two runs took
+    the same branches iff their records are equal (up to hash collisions, which a
+    single flipped decision cannot cause)."""
+    if branches is None:
+        return
+    m = mask.detach().reshape(-1).to(torch.int64)
+    pos = torch.arange(m.numel(), device=m.device, dtype=torch.int64)
+    branches.append((int(m.sum()), int((m * pos).sum())))
+
+
+def decisions_changed(a: list, b: list) -> int:
+    """Number of decision records that differ between two runs' branch records."""
+    return sum(x != y for x, y in zip(a, b)) + abs(len(a) - len(b))
+
+
 # ---------------------------------------------------------------- stencil helpers
 def _px(t, lo, hi):
     return Fn.pad(t, (0, 0, 0, 0, lo, hi))
@@ -278,7 +295,7 @@ class FaceSolve(torch.autograd.Function):
 
 
 # ---------------------------------------------------------------- pressure step
-def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg):
+def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg, branches=None):
     """simulation.py:867-902, 1546-1620, 1622-1660: the face-form operator and rhs
     at the previous state, plus what the transport step needs."""
     f = case.fluid
@@ -289,6 +306,8 @@ def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg):
     mob_t = mob_w + mob_o + mob_g
     dpx, dpy, dpz = p[:-1] - p[1:], p[:, :-1] - p[:, 1:], p[:, :, :-1] - p[:, :, 1:]
     ux, uy, uz = (dpx > 0).detach(), (dpy > 0).detach(), (dpz > 0).detach()
+    for m in (ux, uy, uz):
+        _record(branches, m)
     xl, xh, yl, yh, zl, zh = _faces(mob_t)
     tx, ty, tz = Tx * torch.where(ux, xl, xh), Ty * torch.where(uy, yl, yh), Tz * torch.where(uz, zl, zh)
     n = p.numel()
@@ -306,6 +325,8 @@ def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg):
                 preq = pp[c] - rate / denom
                 pmin = float(w["bhp_min"]) * 1e6
                 if float(preq.detach()) < pmin:
+                    if branches is not None:
+                        branches.append(("bhp_min", wn))
                     term = w["wi"] * mt[c]
                     terms.append(term)
                     bhp_diag = bhp_diag.index_add(0, idx, term.reshape(1))
@@ -349,14 +370,16 @@ def _mx(a):
     return torch.clamp(a, min=1e-12)
 
 
-def _choke(alpha, up, dn, dS):
+def _choke(alpha, up, dn, dS, branches=None):
     """min(alpha, up / max(dS, 1e-12)) when dS > 0, else min(alpha, dn / max(|dS|, 1e-12))."""
-    if float(dS.detach()) > 0.0:
-        return torch.minimum(alpha, up / _mx(dS))
-    return torch.minimum(alpha, dn / _mx(torch.abs(dS)))
+    pos = float(dS.detach()) > 0.0
+    cand = up / _mx(dS) if pos else dn / _mx(torch.abs(dS))
+    if branches is not None:
+        branches.append(("choke", pos, float(cand.detach()) < float(alpha.detach())))
+    return torch.minimum(alpha, cand)
 
 
-def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_old, ps):
+def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_old, ps, branches=None):
     """simulation.py:968-1544 (row-balance form of the BHP rate): new saturations."""
     f = case.fluid
     Tx, Ty, Tz = T
@@ -367,6 +390,8 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
     mob_t = mob_w + mob_o + mob_g
     dpx, dpy, dpz = P[:-1] - P[1:], P[:, :-1] - P[:, 1:], P[:, :, :-1] - P[:, :, 1:]
     ux, uy, uz = (dpx > 0).detach(), (dpy > 0).detach(), (dpz > 0).detach()
+    for m in (ux, uy, uz):
+        _record(branches, m)
     wl = _faces(mob_w)
     gl = _faces(mob_g)
     mwx, mwy, mwz = torch.where(ux, wl[0], wl[1]), torch.where(uy, wl[2], wl[3]), torch.where(uz, wl[4], wl[5])
@@ -421,7 +446,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                 else:
                     up, dn = torch.clamp((1.0 - so_r - sg) - sw, min=0.0), sw - sw_cr
                 dS = (case.dt / _mx(pvf[c])) * q_total
-                alpha = _choke(one, up, dn, dS)
+                alpha = _choke(one, up, dn, dS, branches)
                 q_total = q_total * torch.clamp(alpha, 0.0, 1.0)
                 if w["gas"]:
                     q_g = add(q_g, q_total)
@@ -447,7 +472,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                     sg_up, sg_dn = torch.clamp((1.0 - so_r - sw) - sg, min=0.0), torch.clamp(sg - sg_cr, min=0.0)
                     dSw = (case.dt / _mx(pvf[c])) * (q_total * fw_)
                     dSg = (case.dt / _mx(pvf[c])) * (q_total * fg_)
-                    alpha = _choke(_choke(one, sw_up, sw_dn, dSw), sg_up, sg_dn, dSg)
+                    alpha = _choke(_choke(one, sw_up, sw_dn, dSw, branches), sg_up, sg_dn, dSg, branches)
                     q_total = q_total * torch.clamp(alpha, 0.0, 1.0)
                     lim = w["limits"]
                     if lim:
@@ -463,7 +488,10 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                             m = cand[0]
                             for cnd in cand[1:]:
                                 m = torch.minimum(m, cnd)
-                            if float(torch.abs(q_total).detach()) > float(m.detach()):
+                            limited = float(torch.abs(q_total).detach()) > float(m.detach())
+                            if branches is not None:
+                                branches.append(("limit", wn, limited))
+                            if limited:
                                 q_total = -m
                 q_w = add(q_w, q_total * fw_)
                 q_g = add(q_g, q_total * fg_)
@@ -501,18 +529,24 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
         sg_u = sg + dSg / N
         sw = torch.minimum(torch.clamp(sw_u, min=sw_cr), 1.0 - so_r - sg_u)
         sg = torch.minimum(torch.clamp(sg_u, min=sg_cr), 1.0 - so_r - sw)
+        if branches is not None:  # active clamps of this substep
+            _record(branches, sw_u < sw_cr)
+            _record(branches, torch.clamp(sw_u, min=sw_cr) > 1.0 - so_r - sg_u)
+            _record(branches, sg_u < sg_cr)
+            _record(branches, torch.clamp(sg_u, min=sg_cr) > 1.0 - so_r - sw)
     return sw, sg, N
 
 
-def step(case: ThreePhaseCase, lnk, values, p, sw, sg, solve, direct: bool = False):
-    """One IMPES step; ``direct``: solve is itself differentiable (dense reference)."""
+def step(case: ThreePhaseCase, lnk, values, p, sw, sg, solve, direct: bool = False, branches=None):
+    """One IMPES step; ``direct``: solve is itself differentiable (dense reference);
+    ``branches``: a list that receives the record of every piecewise decision."""
     T = transmissibilities(case, lnk)
-    ps = pressure_system(case, T, values, p, sw, sg)
+    ps = pressure_system(case, T, values, p, sw, sg, branches)
     if direct:
         p_new = solve(ps["diag"], ps["ox"], ps["oy"], ps["oz"], ps["b"])
     else:
         p_new = FaceSolve.apply(ps["diag"], ps["ox"], ps["oy"], ps["oz"], ps["b"], solve)
-    sw_new, sg_new, N = saturation_step(case, T, values, p, p_new, sw, sg, ps)
+    sw_new, sg_new, N = saturation_step(case, T, values, p, p_new, sw, sg, ps, branches)
     return p_new, sw_new, sg_new, N
 
 
@@ -524,10 +558,12 @@ def oil_in_place(case: ThreePhaseCase, p, sw, sg):
     return (case.pv * (so / _mx(case.pvt("Bo", p)) + sg * case.pvt("Rv", p) / _mx(case.pvt("Bg", p)))).sum()
 
 
-def simulate(case: ThreePhaseCase, lnk, values, solve, steps, checkpoint_every=None, direct=False):
+def simulate(case: ThreePhaseCase, lnk, values, solve, steps, checkpoint_every=None, direct=False, branches=None):
     """Forward run from the initial state; returns the final (p, sw, sg) and the
     substep counts. With checkpoint_every, each segment of that many steps keeps
-    only its input state and is recomputed in the reverse sweep."""
+    only its input state and is recomputed in the reverse sweep. ``branches`` (a
+    list, runs without checkpointing only) receives the decision record of every
+    step: equal records of two runs mean the same smooth branch."""
     from torch.utils.checkpoint import checkpoint
 
     p, sw, sg = case.initial_state()
@@ -535,7 +571,7 @@ def simulate(case: ThreePhaseCase, lnk, values, solve, steps, checkpoint_every=N
 
     def segment(n, p, sw, sg, lnk, values):
         for _ in range(n):
-            p, sw, sg, N = step(case, lnk, values, p, sw, sg, solve, direct)
+            p, sw, sg, N = step(case, lnk, values, p, sw, sg, solve, direct, branches)
             counts.append(N)
         return p, sw, sg
 

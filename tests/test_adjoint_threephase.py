@@ -73,9 +73,9 @@ def test_checkpointed_gradient_equals_plain(case):
     assert (gv1 - gv2).norm() <= 1e-12 * gv1.norm()
 
 
-def _J(case, lnk, values):
+def _J(case, lnk, values, branches=None):
     with torch.no_grad():
-        p, sw, sg, counts = tp.simulate(case, lnk, values, scipy_solver(), STEPS)
+        p, sw, sg, counts = tp.simulate(case, lnk, values, scipy_solver(), STEPS, branches=branches)
         return tp.oil_in_place(case, p, sw, sg).item(), counts
 
 
@@ -87,9 +87,12 @@ def test_gradient_matches_finite_differences(case):
     for _ in range(4):
         v = torch.tensor(rng.standard_normal(case.dims))
         v /= v.norm()
-        jp, cp = _J(case, case.lnk0 + h * v, case.rates0)
-        jm, cm = _J(case, case.lnk0 - h * v, case.rates0)
-        assert cp == cm == counts  # no substep count crossed: the derivative is defined
+        bp, bm = [], []
+        jp, cp = _J(case, case.lnk0 + h * v, case.rates0, bp)
+        jm, cm = _J(case, case.lnk0 - h * v, case.rates0, bm)
+        # no substep count crossed; clamp decisions may flip where a clamp is only
+        # marginally active (a kink too small to show at this tolerance)
+        assert cp == cm == counts and len(bp) == len(bm) > 0
         fd = (jp - jm) / (2 * h)
         assert abs(fd - float((gl * v).sum())) <= 1e-4 * abs(fd)
     for wn in range(len(case.wells)):
