@@ -130,6 +130,9 @@ struct Sel<double> {
   static void spmv_add(const CsrF64& A, const double* x, double* y, int g, hipStream_t s) {
     launch_spmv_add_f64(A, x, y, g, s);
   }
+  static void spmv_add_to(const CsrF64& A, const double* x, const double* y0, double* y, int g, hipStream_t s) {
+    launch_spmv_add_to_f64(A, x, y0, y, g, s);
+  }
   static void copy(double* y, const double* x, long long n, hipStream_t s) { launch_copy_f64(y, x, n, s); }
   static void rj0(const CsrF64& R, const double* r, const double* ir, double w, double* bc, double* xc, int g,
                   hipStream_t s) { launch_restrict_j0_f64(R, r, ir, w, bc, xc, g, s); }
@@ -159,6 +162,9 @@ struct Sel<float> {
   static void spmv(const CsrF32& A, const float* x, float* y, int g, hipStream_t s) { launch_spmv_f32(A, x, y, g, s); }
   static void spmv_add(const CsrF32& A, const float* x, float* y, int g, hipStream_t s) {
     launch_spmv_add_f32(A, x, y, g, s);
+  }
+  static void spmv_add_to(const CsrF32& A, const float* x, const float* y0, float* y, int g, hipStream_t s) {
+    launch_spmv_add_to_f32(A, x, y0, y, g, s);
   }
   static void copy(float* y, const float* x, long long n, hipStream_t s) { launch_copy_f32(y, x, n, s); }
   static void rj0(const CsrF32& R, const float* r, const float* ir, double w, float* bc, float* xc, int g,
@@ -661,13 +667,19 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
     Sel<T>::spmv(Sel<T>::R(L), r, Sel<T>::b(C), L.rgroup, s);
     vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s);
   }
+  // Post-smoothing alternates between x and t: prolongate into the buffer from
+  // which post_smooth sweeps end in x, so the cycle needs no final copy.
+  T* target = (prm_.post_smooth % 2 == 0) ? x : Sel<T>::t(L);
   if (zero) {
     // no pre-smoothing happened: x = P e_c
-    Sel<T>::spmv(Sel<T>::P(L), Sel<T>::x(C), cur, L.pgroup, s);
+    Sel<T>::spmv(Sel<T>::P(L), Sel<T>::x(C), target, L.pgroup, s);
     zero = false;
-  } else {
+  } else if (target == cur) {
     Sel<T>::spmv_add(Sel<T>::P(L), Sel<T>::x(C), cur, L.pgroup, s);
+  } else {
+    Sel<T>::spmv_add_to(Sel<T>::P(L), Sel<T>::x(C), cur, target, L.pgroup, s);
   }
+  cur = target;
   other = (cur == x) ? Sel<T>::t(L) : x;
   smooth<T>(l, b, cur, other, prm_.post_smooth, zero, s);
   if (cur != x) Sel<T>::copy(x, cur, L.n, s);

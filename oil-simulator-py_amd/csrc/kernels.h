@@ -63,6 +63,9 @@ void launch_spmv_f32(const CsrF32& A, const float* x, float* y, int group, hipSt
 // y += A x
 void launch_spmv_add_f64(const CsrF64& A, const double* x, double* y, int group, hipStream_t s);
 void launch_spmv_add_f32(const CsrF32& A, const float* x, float* y, int group, hipStream_t s);
+// y = y0 + A x  (y0 and y distinct)
+void launch_spmv_add_to_f64(const CsrF64& A, const double* x, const double* y0, double* y, int group, hipStream_t s);
+void launch_spmv_add_to_f32(const CsrF32& A, const float* x, const float* y0, float* y, int group, hipStream_t s);
 // r = b - A x
 void launch_residual_f64(const CsrF64& A, const double* x, const double* b, double* r, int group, hipStream_t s);
 void launch_residual_f32(const CsrF32& A, const float* x, const float* b, float* r, int group, hipStream_t s);

@@ -19,7 +19,8 @@ constexpr int kBlock = 256;
 
 // kRestrictJ0: y = A x (a restriction R r) and y2 = w * aux .* y (the first
 // L1-Jacobi sweep of the coarse level from a zero guess, aux = its inv_relax)
-enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5, kRestrictJ0 = 6 };
+// kYPlusB: y = b + A x (an add into another buffer; same arithmetic as kYAdd)
+enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5, kRestrictJ0 = 6, kYPlusB = 7 };
 
 template <int G, class T, class TA, int MODE>
 __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -41,6 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
     if (active && g == 0) {
       if (MODE == kY) y[row] = sum;
       else if (MODE == kYAdd) y[row] += sum;
+      else if (MODE == kYPlusB) y[row] = b[row] + sum;
       else if (MODE == kRes) y[row] = b[row] - sum;
       else if (MODE == kJac) y[row] = x[row] + w * (T)aux[row] * (b[row] - sum);
       else if (MODE == kRowScaled) y[row] = (T)aux[row] * sum;
@@ -192,6 +194,12 @@ void launch_spmv_add_f64(const CsrF64& A, const double* x, double* y, int G, hip
 }
 void launch_spmv_add_f32(const CsrF32& A, const float* x, float* y, int G, hipStream_t s) {
   csr_dispatch<float, float, kYAdd>(A.n_rows, A.rp, A.ci, A.v, x, nullptr, (const float*)nullptr, 0.f, y, G, s);
+}
+void launch_spmv_add_to_f64(const CsrF64& A, const double* x, const double* y0, double* y, int G, hipStream_t s) {
+  csr_dispatch<double, double, kYPlusB>(A.n_rows, A.rp, A.ci, A.v, x, y0, (const double*)nullptr, 0.0, y, G, s);
+}
+void launch_spmv_add_to_f32(const CsrF32& A, const float* x, const float* y0, float* y, int G, hipStream_t s) {
+  csr_dispatch<float, float, kYPlusB>(A.n_rows, A.rp, A.ci, A.v, x, y0, (const float*)nullptr, 0.f, y, G, s);
 }
 void launch_residual_f64(const CsrF64& A, const double* x, const double* b, double* r, int G, hipStream_t s) {
   csr_dispatch<double, double, kRes>(A.n_rows, A.rp, A.ci, A.v, x, b, (const double*)nullptr, 0.0, r, G, s);
