@@ -152,6 +152,11 @@ class Hierarchy {
  private:
   void setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s);
   void equilibrate(const double* v_unscaled, bool decide, hipStream_t s);
+  // equil_ (when deciding) and dh_ from the unscaled fine diagonal
+  void equil_factors(const double* diag, bool decide, hipStream_t s);
+  // Galerkin products through the frozen interpolation, relaxation weights and
+  // the coarsest inverse; fine_done: level 0's weights and stencil are current
+  void update_levels(bool fine_done, hipStream_t s);
   void build_levels(hipStream_t s);
   void finish_level_values(int l, hipStream_t s);
   void build_coarsest(int l, hipStream_t s);

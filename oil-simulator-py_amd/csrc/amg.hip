@@ -99,6 +99,88 @@ __global__ void k_st7_scale(int nx, int ny, int nz, const double* __restrict__ d
   }
 }
 
+// Per-step refresh of level 0 from a new stencil, one pass: the equilibrated CSR
+// values (a_ij * dh_i * dh_j, as k_sym_scale forms them), the L1 relaxation
+// weights (the row sum folded in k_diag_relax's eight-lane order, so the weights
+// equal that kernel's bit for bit), the equilibrated stencil and its FP32 copy
+// (as k_st7_scale), the input copies, and the preconditioner's scalings. Each
+// block stages the CSR values of its 256 rows in LDS and writes their contiguous
+// range coalesced.
+struct St7FineUpdate {
+  int nx, ny, nz;
+  const double *d, *ox, *oy, *oz;  // unscaled stencil, as given
+  const double* dh;                // diagonal equilibration factors (null: every factor is a)
+  double a;
+  const int* rp;
+  double *st_d, *st_x, *st_y, *st_z, *diag0;
+  double* v;
+  double* ir64;
+  float* ir32;
+  double *sd, *sx, *sy, *sz;
+  float *sd32, *sx32, *sy32, *sz32;
+  double *dh_out, *sin, *sout;     // dh_out written only when every factor is a
+};
+
+constexpr int kFineRows = 256;
+
+template <bool kScalar>
+__global__ __launch_bounds__(kFineRows) void k_st7_fine_update(St7FineUpdate u) {
+  __shared__ double stage[kFineRows * 7];
+  const long long n = (long long)u.nx * u.ny * u.nz;
+  const long long sx = (long long)u.ny * u.nz, sy = u.nz;
+  for (long long c0 = (long long)blockIdx.x * kFineRows; c0 < n; c0 += (long long)gridDim.x * kFineRows) {
+    const long long c = c0 + threadIdx.x;
+    const long long c1 = (c0 + kFineRows < n) ? c0 + kFineRows : n;
+    const int base = u.rp[c0];
+    if (c < n) {
+      int i, j, k;
+      cell_ijk(c, u.ny, u.nz, i, j, k);
+      const double h = kScalar ? u.a : u.dh[c];
+      const double dc = u.d[c], xc = u.ox[c], yc = u.oy[c], zc = u.oz[c];
+      // CSR order of a row: -x, -y, -z, diagonal, +z, +y, +x
+      const bool pr[7] = {i > 0, j > 0, k > 0, true, k < u.nz - 1, j < u.ny - 1, i < u.nx - 1};
+      double e[7];
+      e[0] = pr[0] ? u.ox[c - sx] * h * (kScalar ? u.a : u.dh[c - sx]) : 0.0;
+      e[1] = pr[1] ? u.oy[c - sy] * h * (kScalar ? u.a : u.dh[c - sy]) : 0.0;
+      e[2] = pr[2] ? u.oz[c - 1] * h * (kScalar ? u.a : u.dh[c - 1]) : 0.0;
+      e[3] = dc * h * h;
+      e[4] = pr[4] ? zc * h * (kScalar ? u.a : u.dh[c + 1]) : 0.0;
+      e[5] = pr[5] ? yc * h * (kScalar ? u.a : u.dh[c + sy]) : 0.0;
+      e[6] = pr[6] ? xc * h * (kScalar ? u.a : u.dh[c + sx]) : 0.0;
+      // |a_ij| of the row's m-th entry sits in lane m of k_diag_relax's group
+      double l[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int m = 0;
+#pragma unroll
+      for (int t = 0; t < 7; ++t) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (pr[t] && m == q) l[q] = fabs(e[t]);
+        m += pr[t] ? 1 : 0;
+      }
+      const double rs = ((l[0] + l[4]) + (l[2] + l[6])) + ((l[1] + l[5]) + (l[3] + l[7]));
+      double den = fmax(fabs(e[3]), 0.3 * rs);
+      den = fmax(den, 1e-30);
+      const double ir = fmin(1.0 / den, 1e2);
+      u.ir64[c] = ir;
+      u.ir32[c] = (float)ir;
+      int p = u.rp[c] - base;
+#pragma unroll
+      for (int t = 0; t < 7; ++t)
+        if (pr[t]) stage[p++] = e[t];
+      u.st_d[c] = dc; u.st_x[c] = xc; u.st_y[c] = yc; u.st_z[c] = zc; u.diag0[c] = dc;
+      u.sd[c] = e[3]; u.sx[c] = e[6]; u.sy[c] = e[5]; u.sz[c] = e[4];
+      u.sd32[c] = (float)e[3]; u.sx32[c] = (float)e[6]; u.sy32[c] = (float)e[5]; u.sz32[c] = (float)e[4];
+      if (kScalar) u.dh_out[c] = h;
+      u.sin[c] = h * dc;
+      u.sout[c] = h;
+    }
+    __syncthreads();
+    const int cnt = u.rp[c1] - base;
+    for (int q = threadIdx.x; q < cnt; q += kFineRows) u.v[base + q] = stage[q];
+    __syncthreads();
+  }
+}
+
 __global__ void k_set_cf(signed char* cf, int idx) { cf[idx] = kC; }
 __global__ void k_neg_flags(int m, const int* __restrict__ tp, int* __restrict__ out) {
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < m; e += (long long)gridDim.x * blockDim.x)
@@ -286,19 +368,13 @@ void Hierarchy::setup_stencil(int nx, int ny, int nz, const double* d, const dou
   setup_impl((int)n, nnz, rp.p, ci.p, v.p, s);
 }
 
-void Hierarchy::equilibrate(const double* vu, bool decide, hipStream_t s) {
-  Level& L0 = levels_[0];
-  const int n = L0.n;
-  diag0_.alloc(n);
-  dh_.alloc(n);
-  sin_.alloc(n);
-  sout_.alloc(n);
-  launch_diag_relax(n, L0.rp.p, L0.ci.p, vu, diag0_.p, nullptr, nullptr, s);
+void Hierarchy::equil_factors(const double* diag, bool decide, hipStream_t s) {
+  const int n = levels_[0].n;
   if (prm_.equil_mode == kEquilScalar) {
     if (decide) {
       // lower median of |a_ii| (torch.median semantics), chosen once per setup
       std::vector<double> d((size_t)n);
-      IMPES_HIP_CHECK(hipMemcpyAsync(d.data(), diag0_.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      IMPES_HIP_CHECK(hipMemcpyAsync(d.data(), diag, sizeof(double) * n, hipMemcpyDeviceToHost, s));
       IMPES_HIP_CHECK(hipStreamSynchronize(s));
       for (auto& x : d) x = std::max(std::fabs(x), 1e-30);
       auto mid = d.begin() + (n - 1) / 2;
@@ -307,24 +383,32 @@ void Hierarchy::equilibrate(const double* vu, bool decide, hipStream_t s) {
       scalar_ = equil_ ? 1.0 / std::sqrt(*mid) : 1.0;
     }
     launch_fill_f64(dh_.p, scalar_, n, s);
-    launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
   } else {
     DevBuf<int> flags;
     flags.alloc(n);
-    launch_count_small_diag(n, diag0_.p, prm_.equil_threshold, flags.p, s);
+    launch_count_small_diag(n, diag, prm_.equil_threshold, flags.p, s);
     // lower median (torch.median semantics) below the threshold  <=>  at least
     // (n-1)/2 + 1 entries below it
     const long long small = sum_i32(flags.p, n, s);
     equil_ = small >= (long long)((n - 1) / 2 + 1);
-    if (equil_) {
-      launch_dhalf_inv(n, diag0_.p, dh_.p, s);
-      launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
-    } else {
-      launch_fill_f64(dh_.p, 1.0, n, s);
-      d2d(L0.v.p, vu, L0.nnz, s);
-    }
+    if (equil_) launch_dhalf_inv(n, diag, dh_.p, s);
+    else launch_fill_f64(dh_.p, 1.0, n, s);
     IMPES_HIP_CHECK(hipStreamSynchronize(s));  // flags freed on return
   }
+}
+
+void Hierarchy::equilibrate(const double* vu, bool decide, hipStream_t s) {
+  Level& L0 = levels_[0];
+  const int n = L0.n;
+  diag0_.alloc(n);
+  dh_.alloc(n);
+  sin_.alloc(n);
+  sout_.alloc(n);
+  launch_diag_relax(n, L0.rp.p, L0.ci.p, vu, diag0_.p, nullptr, nullptr, s);
+  equil_factors(diag0_.p, decide, s);
+  // unequilibrated: every factor is 1 and the scaled values equal the given ones
+  if (equil_) launch_sym_scale(n, L0.rp.p, L0.ci.p, vu, dh_.p, L0.v.p, s);
+  else d2d(L0.v.p, vu, L0.nnz, s);
   launch_scale_f64(sin_.p, dh_.p, diag0_.p, n, s);
   launch_copy_f64(sout_.p, dh_.p, n, s);
 }
@@ -545,6 +629,10 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "amg update before setup");
   if (v != v0_unscaled_.p) d2d(v0_unscaled_.p, v, levels_[0].nnz, s);
   equilibrate(v0_unscaled_.p, false, s);
+  update_levels(false, s);
+}
+
+void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
   for (int l = 0; l + 1 < (int)levels_.size(); ++l) {
     Level& A = levels_[l];
     if (A.coarsest) break;
@@ -554,8 +642,10 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
   }
   for (int l = 0; l < (int)levels_.size(); ++l) {
     Level& L = levels_[l];
-    launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
-    launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    if (!(fine_done && l == 0)) {
+      launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
+      launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    }
     if (L.coarsest) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
       if (L.dense) {
@@ -564,16 +654,26 @@ void Hierarchy::update_csr_values(const double* v, hipStream_t s) {
       }
     }
   }
-  if (stencil_) refresh_stencil(s);
+  if (stencil_ && !fine_done) refresh_stencil(s);
 }
 
 void Hierarchy::update_stencil(const double* d, const double* ox, const double* oy, const double* oz, hipStream_t s) {
-  IMPES_REQUIRE(stencil_, "amg update_stencil on a CSR hierarchy");
+  IMPES_REQUIRE(stencil_ && !levels_.empty(), "amg update_stencil on a CSR hierarchy");
   const long long n = (long long)nx_ * ny_ * nz_;
-  d2d(st_d_.p, d, n, s); d2d(st_x_.p, ox, n, s); d2d(st_y_.p, oy, n, s); d2d(st_z_.p, oz, n, s);
   Level& L0 = levels_[0];
-  launch_st7_to_csr_values(nx_, ny_, nz_, d, ox, oy, oz, L0.rp.p, v0_unscaled_.p, s);
-  update_csr_values(v0_unscaled_.p, s);
+  // Level 0 runs on the stencil kernels: its relaxation weights and equilibrated
+  // stencil come from the fused pass, and its CSR values only feed the Galerkin
+  // products (the FP32 copy of level-0 CSR values is not read in this mode).
+  const bool scalar = prm_.equil_mode == kEquilScalar;
+  if (!scalar) equil_factors(d, false, s);
+  St7FineUpdate u{nx_, ny_, nz_, d, ox, oy, oz, scalar ? nullptr : dh_.p, scalar_, L0.rp.p,
+                  st_d_.p, st_x_.p, st_y_.p, st_z_.p, diag0_.p, L0.v.p, L0.ir64.p, L0.ir32.p,
+                  sd_.p, sx_.p, sy_.p, sz_.p, sd32_.p, sx32_.p, sy32_.p, sz32_.p, dh_.p, sin_.p, sout_.p};
+  const int blocks = (int)std::min<long long>((n + kFineRows - 1) / kFineRows, 1 << 16);
+  if (scalar) klaunch(k_st7_fine_update<true>, dim3(blocks), dim3(kFineRows), 0, s, u);
+  else klaunch(k_st7_fine_update<false>, dim3(blocks), dim3(kFineRows), 0, s, u);
+  IMPES_LAUNCH_CHECK();
+  update_levels(true, s);
 }
 
 bool Hierarchy::coarse_pinned(hipStream_t s) const {

@@ -12,16 +12,6 @@ namespace impes {
 
 namespace {
 
-__global__ void k_dense_identity(int n, double* __restrict__ aug) {
-  // aug is n x 2n: [A_pinned | I]
-  const long long w = 2LL * n;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < (long long)n * w;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int r = (int)(t / w), c = (int)(t % w);
-    aug[t] = (c >= n && c - n == r) ? 1.0 : 0.0;
-  }
-}
-
 // One workgroup: pin = mode==1 || (mode==2 && no row has |row sum| > 1e-8 |a_ii|).
 __global__ __launch_bounds__(256) void k_pin_decide(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                     const double* __restrict__ v, int mode, int* __restrict__ pin) {
@@ -45,18 +35,51 @@ __global__ void k_pin_rhs32(const int* __restrict__ pin, float* __restrict__ b) 
   if (*pin) b[0] = 0.f;
 }
 
-__global__ void k_dense_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                const double* __restrict__ v, const int* __restrict__ pin_flag,
-                                double* __restrict__ aug) {
+// One workgroup per row r of [A_pinned | I]: the row's entries pass through LDS
+// in chunks and each thread sums the entries of its (at most four) columns in
+// entry order, so repeated column indices add up as in a serial scatter. Pinned
+// DOF 0: row and column 0 zero but a_00 = 1.
+constexpr int kDenseMax = 512;
+constexpr int kFillThreads = 256;
+constexpr int kFillPer = 2 * kDenseMax / kFillThreads;
+__global__ __launch_bounds__(kFillThreads) void k_dense_fill(int n, const int* __restrict__ rp,
+                                                             const int* __restrict__ ci, const double* __restrict__ v,
+                                                             const int* __restrict__ pin_flag,
+                                                             double* __restrict__ aug) {
+  __shared__ int lc[kDenseMax];
+  __shared__ double lv[kDenseMax];
+  const int r = blockIdx.x;
   const long long w = 2LL * n;
   const int pin = *pin_flag;
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
-    for (int e = rp[r]; e < rp[r + 1]; ++e) {
-      const int c = ci[e];
-      if (pin && (r == 0 || c == 0)) continue;  // pinned DOF 0
-      aug[r * w + c] += v[e];
+  const int e0 = rp[r], m = rp[r + 1] - e0;
+  double acc[kFillPer];
+#pragma unroll
+  for (int q = 0; q < kFillPer; ++q) acc[q] = 0.0;
+  for (int t0 = 0; t0 < m; t0 += kDenseMax) {
+    const int mc = (m - t0 < kDenseMax) ? m - t0 : kDenseMax;
+    __syncthreads();  // the previous chunk is consumed
+    for (int t = threadIdx.x; t < mc; t += kFillThreads) {
+      lc[t] = ci[e0 + t0 + t];
+      lv[t] = v[e0 + t0 + t];
     }
-    if (pin && r == 0) aug[0] = 1.0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kFillPer; ++q) {
+      const int c = threadIdx.x + q * kFillThreads;
+      if (c >= n || (pin && (r == 0 || c == 0))) continue;
+      for (int t = 0; t < mc; ++t)
+        if (lc[t] == c) acc[q] += lv[t];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kFillPer; ++q) {
+    const int c = threadIdx.x + q * kFillThreads;
+    if (c >= 2 * n) continue;
+    double a;
+    if (c >= n) a = (c - n == r) ? 1.0 : 0.0;
+    else if (pin && (r == 0 || c == 0)) a = (r == 0 && c == 0) ? 1.0 : 0.0;
+    else a = acc[q];
+    aug[r * w + c] = a;
   }
 }
 
@@ -218,15 +241,45 @@ __global__ __launch_bounds__(1024) void k_sweep_regs(int n, double* __restrict__
     for (int a = 0; a < R; ++a) ci[a] = col[ti + 32 * a];
 #pragma unroll
     for (int b = 0; b < R; ++b) cj[b] = col[tj + 32 * b];
+    if constexpr (R >= 5) {
+      // at 5 x 5 slots the split form below no longer fits 128 VGPRs
 #pragma unroll
-    for (int a = 0; a < R; ++a) {
-      const int i = ti + 32 * a;
+      for (int a = 0; a < R; ++a) {
+        const int i = ti + 32 * a;
 #pragma unroll
-      for (int b = 0; b < R; ++b) {
-        const int j = tj + 32 * b;
-        if (i == k && j == k) T[a][b] = -inv;
-        else if (i == k || j == k) T[a][b] *= inv;
-        else T[a][b] -= ci[a] * cj[b] * inv;
+        for (int b = 0; b < R; ++b) {
+          const int j = tj + 32 * b;
+          if (i == k && j == k) T[a][b] = -inv;
+          else if (i == k || j == k) T[a][b] *= inv;
+          else T[a][b] -= ci[a] * cj[b] * inv;
+        }
+      }
+    } else {
+      // Every entry takes the off-pivot update, then the owners of row and column
+      // k overwrite theirs with T_ik * inv = ci * inv and T_kj * inv = cj * inv (the
+      // matrix stays bit-symmetric), and -inv on the pivot: the same values as the
+      // per-entry form above. Branch-free for all but one half-wave row and one lane
+      // per half-wave, so a pivot costs one product and one multiply-add per entry.
+#pragma unroll
+      for (int a = 0; a < R; ++a)
+#pragma unroll
+        for (int b = 0; b < R; ++b) T[a][b] -= ci[a] * cj[b] * inv;
+      const int kk = k & 31, bk = k >> 5;
+      if (tj == kk) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b)
+            if (b == bk) T[a][b] = (ti + 32 * a == k) ? -inv : ci[a] * inv;
+      }
+      if (ti == kk) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+          if (a == bk) {
+#pragma unroll
+            for (int b = 0; b < R; ++b)
+              if (tj + 32 * b != k) T[a][b] = cj[b] * inv;
+          }
       }
     }
   }
@@ -339,10 +392,7 @@ void launch_pin_rhs_f32(const int* pin, float* b, hipStream_t s) {
 void launch_csr_to_dense(int n, const int* rp, const int* ci, const double* v, const int* pin, double* aug,
                          hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= 512, "dense coarsest solve supports 1 <= n <= 512");
-  const long long tot = 2LL * n * n;
-  k_dense_identity<<<grid_for(tot, 256), 256, 0, s>>>(n, aug);
-  IMPES_LAUNCH_CHECK();
-  k_dense_scatter<<<grid_for(n, 256), 256, 0, s>>>(n, rp, ci, v, pin, aug);
+  k_dense_fill<<<n, kFillThreads, 0, s>>>(n, rp, ci, v, pin, aug);
   IMPES_LAUNCH_CHECK();
 }
 

@@ -9,6 +9,9 @@
 // Stencil7: the level-0 IMPES pressure operator is a symmetric 7-point stencil,
 // so it is stored index-free as one diagonal and three face-coefficient arrays
 // (4 values per cell instead of CSR's ~7 values + 7 indices + row pointers).
+#include <cstdint>
+#include <type_traits>
+
 #include "kernels.h"
 #include "ktimer.h"
 
@@ -32,6 +35,15 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
   for (long long base = xcd_block() * RPB; base < n; base += (long long)gridDim.x * RPB) {
     const long long row = base + threadIdx.x / G;
     const bool active = row < n;
+    // the row's own operands are loaded before the row walk, so their latency
+    // overlaps the (row pointer -> column -> gather) chain instead of following it
+    T ob = T(0), ox = T(0), oy = T(0), oa = T(0);
+    if (active && g == 0) {
+      if (MODE == kYPlusB || MODE == kRes || MODE == kJac || MODE == kRowScaledRes) ob = b[row];
+      if (MODE == kJac) ox = x[row];
+      if (MODE == kYAdd) oy = y[row];
+      if (MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes || MODE == kRestrictJ0) oa = (T)aux[row];
+    }
     T sum = T(0);
     if (active) {
       const int e1 = rp[row + 1];
@@ -41,15 +53,15 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
     if (G > 1) sum = group_sum<G>(sum);
     if (active && g == 0) {
       if (MODE == kY) y[row] = sum;
-      else if (MODE == kYAdd) y[row] += sum;
-      else if (MODE == kYPlusB) y[row] = b[row] + sum;
-      else if (MODE == kRes) y[row] = b[row] - sum;
-      else if (MODE == kJac) y[row] = x[row] + w * (T)aux[row] * (b[row] - sum);
-      else if (MODE == kRowScaled) y[row] = (T)aux[row] * sum;
-      else if (MODE == kRowScaledRes) y[row] = (T)aux[row] * (b[row] - sum);
+      else if (MODE == kYAdd) y[row] = oy + sum;
+      else if (MODE == kYPlusB) y[row] = ob + sum;
+      else if (MODE == kRes) y[row] = ob - sum;
+      else if (MODE == kJac) y[row] = ox + w * oa * (ob - sum);
+      else if (MODE == kRowScaled) y[row] = oa * sum;
+      else if (MODE == kRowScaledRes) y[row] = oa * (ob - sum);
       else if (MODE == kRestrictJ0) {
         y[row] = sum;
-        y2[row] = w * (T)aux[row] * sum;
+        y2[row] = w * oa * sum;
       }
     }
   }
@@ -111,6 +123,82 @@ __global__ __launch_bounds__(kBlock) void k_st7(Stencil7<T> S, const T* __restri
   }
 }
 
+// Wide form of k_st7 for nz a multiple of V = 16 / sizeof(T): a thread takes V
+// consecutive cells of one z-column (same i, j) and reads every coefficient and
+// x array with 16-byte loads (the +-x and +-y neighbours of V aligned cells are
+// V aligned cells), plus oz[c-1], x[c-1] and x[c+V] for the column ends. Each
+// cell's sum is formed in St7Row's order, so the results equal k_st7's.
+template <class T>
+struct Wide;
+template <>
+struct Wide<float> { using type = float4; };
+template <>
+struct Wide<double> { using type = double2; };
+
+template <class T, int V>
+__device__ __forceinline__ void ld_wide(const T* __restrict__ p, T (&a)[V]) {
+  static_assert(V * sizeof(T) == 16, "16-byte vectors");
+  const typename Wide<T>::type v = *reinterpret_cast<const typename Wide<T>::type*>(p);
+  __builtin_memcpy(a, &v, 16);
+}
+template <class T, int V>
+__device__ __forceinline__ void st_wide(T* __restrict__ p, const T (&a)[V]) {
+  typename Wide<T>::type v;
+  __builtin_memcpy(&v, a, 16);
+  *reinterpret_cast<typename Wide<T>::type*>(p) = v;
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_st7_wide(Stencil7<T> S, const T* __restrict__ x, const T* __restrict__ b,
+                                                     const T* __restrict__ aux, T w, T* __restrict__ y) {
+  constexpr int V = 16 / sizeof(T);
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  const long long sy = S.nz;
+  const long long sx = (long long)S.ny * S.nz;
+  for (long long q = xcd_block() * blockDim.x + threadIdx.x; q * V < n; q += (long long)gridDim.x * blockDim.x) {
+    const long long c = q * V;
+    const int k0 = (int)(c % S.nz);
+    const long long t = c / S.nz;
+    const int j = (int)(t % S.ny);
+    const int i = (int)(t / S.ny);
+    T xc[V], dd[V], fz[V], fy[V], fx[V];
+    T fw[V] = {}, xw[V] = {}, fs[V] = {}, xs[V] = {}, xn[V] = {}, xe[V] = {}, bb[V] = {}, aa[V] = {};
+    ld_wide(x + c, xc);
+    ld_wide(S.d + c, dd);
+    ld_wide(S.oz + c, fz);
+    ld_wide(S.oy + c, fy);
+    ld_wide(S.ox + c, fx);
+    if (i > 0) { ld_wide(S.ox + c - sx, fw); ld_wide(x + c - sx, xw); }
+    if (j > 0) { ld_wide(S.oy + c - sy, fs); ld_wide(x + c - sy, xs); }
+    if (j < S.ny - 1) ld_wide(x + c + sy, xn);
+    if (i < S.nx - 1) ld_wide(x + c + sx, xe);
+    const T zb = k0 > 0 ? S.oz[c - 1] : T(0);
+    const T xb = k0 > 0 ? x[c - 1] : T(0);
+    const T xa = k0 + V < S.nz ? x[c + V] : T(0);
+    if (MODE == kRes || MODE == kJac || MODE == kRowScaledRes) ld_wide(b + c, bb);
+    if (MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes) ld_wide(aux + c, aa);
+    T out[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int k = k0 + m;
+      T sum = T(0);
+      if (i > 0) sum += fw[m] * xw[m];
+      if (j > 0) sum += fs[m] * xs[m];
+      if (k > 0) sum += (m == 0 ? zb : fz[m > 0 ? m - 1 : 0]) * (m == 0 ? xb : xc[m > 0 ? m - 1 : 0]);
+      sum += dd[m] * xc[m];
+      if (k < S.nz - 1) sum += fz[m] * (m == V - 1 ? xa : xc[m < V - 1 ? m + 1 : 0]);
+      if (j < S.ny - 1) sum += fy[m] * xn[m];
+      if (i < S.nx - 1) sum += fx[m] * xe[m];
+      if (MODE == kY) out[m] = sum;
+      else if (MODE == kRes) out[m] = bb[m] - sum;
+      else if (MODE == kJac) out[m] = xc[m] + w * aa[m] * (bb[m] - sum);
+      else if (MODE == kRowScaled) out[m] = aa[m] * sum;
+      else out[m] = aa[m] * (bb[m] - sum);
+    }
+    st_wide(y + c, out);
+  }
+}
+
 // y = dinv .* (A z) and z itself, with z = sout .* x formed on the fly from the
 // V-cycle's output x (FP32 or FP64): the preconditioner's output scaling fused
 // into the Krylov operator apply. Same operation order as k_scale followed by
@@ -143,16 +231,104 @@ __global__ __launch_bounds__(kBlock) void k_st7_rowscaled_z(Stencil7<double> S, 
   }
 }
 
+// Wide form of k_st7_rowscaled_z for even nz: two cells of one z-column per
+// thread, 16-byte loads of every FP64 array (8-byte loads of an FP32 x), the same
+// operation order per cell.
+template <class TX>
+struct Pair;
+template <>
+struct Pair<float> { using type = float2; };
+template <>
+struct Pair<double> { using type = double2; };
+
+template <class TX>
+__device__ __forceinline__ void ld_pair(const TX* __restrict__ p, TX (&a)[2]) {
+  const typename Pair<TX>::type v = *reinterpret_cast<const typename Pair<TX>::type*>(p);
+  a[0] = v.x;
+  a[1] = v.y;
+}
+
+template <class TX>
+__global__ __launch_bounds__(kBlock) void k_st7_rowscaled_z_wide(Stencil7<double> S, const double* __restrict__ dinv,
+                                                                 const double* __restrict__ sout,
+                                                                 const TX* __restrict__ x, double* __restrict__ z,
+                                                                 double* __restrict__ y) {
+  const long long n = (long long)S.nx * S.ny * S.nz;
+  const long long sy = S.nz;
+  const long long sx = (long long)S.ny * S.nz;
+  for (long long q = xcd_block() * blockDim.x + threadIdx.x; q * 2 < n; q += (long long)gridDim.x * blockDim.x) {
+    const long long c = q * 2;
+    const int k0 = (int)(c % S.nz);
+    const long long t = c / S.nz;
+    const int j = (int)(t % S.ny);
+    const int i = (int)(t / S.ny);
+    // z = sout .* x at the cell pair and at each neighbour pair
+    auto zpair = [&](long long p0, double (&zz)[2]) {
+      TX xx[2];
+      double ss[2];
+      ld_pair(x + p0, xx);
+      ld_pair(sout + p0, ss);
+      zz[0] = ss[0] * (double)xx[0];
+      zz[1] = ss[1] * (double)xx[1];
+    };
+    double zc[2], dd[2], fz[2], fy[2], fx[2], di[2];
+    double zw[2] = {}, fw[2] = {}, zs[2] = {}, fs[2] = {}, zn[2] = {}, ze[2] = {};
+    zpair(c, zc);
+    ld_pair(S.d + c, dd);
+    ld_pair(S.oz + c, fz);
+    ld_pair(S.oy + c, fy);
+    ld_pair(S.ox + c, fx);
+    ld_pair(dinv + c, di);
+    if (i > 0) { ld_pair(S.ox + c - sx, fw); zpair(c - sx, zw); }
+    if (j > 0) { ld_pair(S.oy + c - sy, fs); zpair(c - sy, zs); }
+    if (j < S.ny - 1) zpair(c + sy, zn);
+    if (i < S.nx - 1) zpair(c + sx, ze);
+    const double fb = k0 > 0 ? S.oz[c - 1] : 0.0;
+    const double zb = k0 > 0 ? sout[c - 1] * (double)x[c - 1] : 0.0;
+    const double za = k0 + 2 < S.nz ? sout[c + 2] * (double)x[c + 2] : 0.0;
+    double out[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int k = k0 + m;
+      double sum = 0.0;
+      if (i > 0) sum += fw[m] * zw[m];
+      if (j > 0) sum += fs[m] * zs[m];
+      if (k > 0) sum += (m == 0 ? fb : fz[0]) * (m == 0 ? zb : zc[0]);
+      sum += dd[m] * zc[m];
+      if (k < S.nz - 1) sum += fz[m] * (m == 1 ? za : zc[1]);
+      if (j < S.ny - 1) sum += fy[m] * zn[m];
+      if (i < S.nx - 1) sum += fx[m] * ze[m];
+      out[m] = di[m] * sum;
+    }
+    *reinterpret_cast<double2*>(z + c) = make_double2(zc[0], zc[1]);
+    *reinterpret_cast<double2*>(y + c) = make_double2(out[0], out[1]);
+  }
+}
+
 template <class T>
 __global__ void k_jacobi0(const T* __restrict__ b, const T* __restrict__ inv_relax, T w, T* __restrict__ xo, int n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     xo[i] = w * inv_relax[i] * b[i];
 }
 
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 template <class T, class TA, int MODE>
 void st7_launch(const Stencil7<T>& S, const T* x, const T* b, const TA* aux, T w, T* y, hipStream_t s) {
   const long long n = (long long)S.nx * S.ny * S.nz;
   if (n <= 0) return;
+  constexpr int V = 16 / sizeof(T);
+  constexpr bool uses_b = MODE == kRes || MODE == kJac || MODE == kRowScaledRes;
+  constexpr bool uses_aux = MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes;
+  if constexpr (std::is_same<T, TA>::value) {
+    // the wide form needs whole 16-byte vectors along z and 16-byte aligned arrays
+    if (S.nz % V == 0 && aligned16(x) && aligned16(y) && aligned16(S.d) && aligned16(S.ox) && aligned16(S.oy) &&
+        aligned16(S.oz) && (!uses_b || aligned16(b)) && (!uses_aux || aligned16(aux))) {
+      klaunch(k_st7_wide<T, MODE>, dim3(grid_for(n / V, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, x, b, aux, w, y);
+      IMPES_LAUNCH_CHECK();
+      return;
+    }
+  }
   klaunch(k_st7<T, TA, MODE>, dim3(grid_for(n, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, x, b, aux, w, y);
   IMPES_LAUNCH_CHECK();
 }
@@ -278,7 +454,13 @@ static void st7_rowscaled_z(const Stencil7<double>& S, const double* dinv, const
   // x and sout, 4 coefficient arrays and dinv read; z and y written
   KScope ks(sizeof(TX) == 4 ? "st7_rowscaled_z_f32in" : "st7_rowscaled_z_f64in",
             (double)n * (sizeof(TX) + 8 * 8), s);
-  klaunch(k_st7_rowscaled_z<TX>, dim3(grid_for(n, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, dinv, sout, x, z, y);
+  if (S.nz % 2 == 0 && aligned16(dinv) && aligned16(sout) && aligned16(z) && aligned16(y) && aligned16(S.d) &&
+      aligned16(S.ox) && aligned16(S.oy) && aligned16(S.oz) && ((uintptr_t)x % (2 * sizeof(TX))) == 0) {
+    klaunch(k_st7_rowscaled_z_wide<TX>, dim3(grid_for(n / 2, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, dinv, sout, x,
+            z, y);
+  } else {
+    klaunch(k_st7_rowscaled_z<TX>, dim3(grid_for(n, kBlock, 1 << 18)), dim3(kBlock), 0, s, S, dinv, sout, x, z, y);
+  }
   IMPES_LAUNCH_CHECK();
 }
 void launch_st7_rowscaled_spmv_z_f64(const Stencil7<double>& S, const double* dinv, const double* sout,

@@ -53,6 +53,27 @@ def test_spmv_csr_and_stencil(dev, dims):
     assert rel(ys, ref) < 1e-14
 
 
+@pytest.mark.parametrize("dims", [(6, 5, 4), (7, 3, 8), (4, 4, 2), (5, 6, 10)])
+def test_spmv_st7_wide_equals_scalar(dev, dims):
+    """Stencil products over whole 16-byte vectors along z (even nz in FP64)
+    against the one-cell-per-thread form, selected here by an x one element off
+    16-byte alignment: bit-equal, and equal to the CSR product to 1e-14."""
+    import impes_hip as H
+
+    A, d, ox, oy, oz = impes_like(*dims, seed=11)
+    x = np.random.default_rng(6).standard_normal(A.shape[0])
+    xa = t(x, dev)
+    buf = torch.empty(A.shape[0] + 1, dtype=torch.float64, device=dev)
+    xu = buf[1:]
+    xu.copy_(xa)
+    assert xa.data_ptr() % 16 == 0 and xu.data_ptr() % 16 != 0
+    args = (dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev))
+    yw = H.spmv_st7(*args, xa)
+    ys = H.spmv_st7(*args, xu)
+    assert torch.equal(yw, ys)
+    assert rel(yw.cpu().numpy(), A @ x) < 1e-14
+
+
 def test_l1jacobi_and_dot(dev):
     import impes_hip as H
 
@@ -138,17 +159,59 @@ def test_hierarchy_update_values(dev):
     _compare_hierarchy(hg, ho)
 
 
-def test_stencil_hierarchy_equals_csr_hierarchy(dev):
+@pytest.mark.parametrize("dims", [(16, 12, 6), (12, 10, 8)])
+def test_stencil_hierarchy_equals_csr_hierarchy(dev, dims):
+    """nz = 6 runs the FP32 level-0 sweeps one cell per thread, nz = 8 four cells
+    per thread with 16-byte loads (FP64: two cells per thread in both)."""
     import impes_hip as H
 
-    dims = (16, 12, 6)
     A, d, ox, oy, oz = impes_like(*dims, seed=7)
     ho = oa.Hierarchy(A)
     hg = H.AmgHierarchy()
     hg.setup_st7(dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev))
     _compare_hierarchy(hg, ho)
     b = np.random.default_rng(3).standard_normal(A.shape[0])
-    assert rel(hg.vcycle(t(b, dev)).cpu().numpy(), ho.vcycle(b)) < 1e-11
+    xo = ho.vcycle(b)
+    assert rel(hg.vcycle(t(b, dev)).cpu().numpy(), xo) < 1e-11
+    assert rel(hg.vcycle(t(b, dev), precision=32).cpu().numpy(), xo) < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["reference", "default"])
+@pytest.mark.parametrize("coarsest", [120, 200])
+def test_stencil_update_equals_csr_update(dev, mode, coarsest):
+    """A numeric update from a new stencil (one fused pass over level 0) against
+    the oracle's update and, exactly, against the CSR hierarchy's update of the
+    same matrix: every level operator and every relaxation weight bit-equal.
+    The coarsest sizes (53/116/156/160 rows) run each register tile of the
+    coarsest inverse; scalar and diagonal equilibration both scale."""
+    import impes_hip as H
+
+    dims = (16, 12, 6)
+    A1, d1, ox1, oy1, oz1 = impes_like(*dims, seed=9)
+    A2, d2, ox2, oy2, oz2 = impes_like(*dims, seed=9, sigma=1.3)  # same well cells
+    kw = dict(coarsest_size=coarsest)
+    if mode == "reference":
+        kw.update(oa.Hierarchy.REFERENCE)
+    ho = oa.Hierarchy(A1, **kw)
+    ho.update(A2)
+    hs = H.AmgHierarchy(**kw)
+    hs.setup_st7(dims, t(d1, dev), t(ox1, dev), t(oy1, dev), t(oz1, dev))
+    hs.update_st7(t(d2, dev), t(ox2, dev), t(oy2, dev), t(oz2, dev))
+    hc = H.AmgHierarchy(**kw)
+    hc.setup_csr(*to_dev(A1, dev))
+    hc.update_csr(to_dev(A2, dev)[2])
+    assert ho.equilibrated
+    _compare_hierarchy(hs, ho)
+    for l in range(hs.num_levels):
+        es, ec = hs.export_level(l), hc.export_level(l)
+        diff = csr_from_export(es) - csr_from_export(ec)
+        assert diff.nnz == 0 or abs(diff).max() == 0.0, f"level {l} operator"
+        assert torch.equal(es["inv_relax"], ec["inv_relax"]), f"level {l} relaxation weights"
+    b = np.random.default_rng(4).standard_normal(A1.shape[0])
+    assert rel(hs.vcycle(t(b, dev), precision=64).cpu().numpy(), ho.vcycle(b)) < 1e-11
+    v = t(b, dev)
+    zs, zc = hs.precondition(v, precision=64), hc.precondition(v, precision=64)
+    assert rel(zs.cpu().numpy(), zc.cpu().numpy()) < 1e-12
 
 
 @pytest.mark.parametrize("precision", [64, 32])
