@@ -348,14 +348,19 @@ double st7_bytes(const Stencil7<T>& S, int val_bytes, int vecs) {
   return n * val_bytes * (4 + 1 + vecs);
 }
 
-// Lanes per row: the smallest power of two with at most ~4 entries per lane. A
-// lane then has several independent (value, column, gather) loads in flight,
-// which is what keeps HBM busy: with one entry per lane the sweep is latency-bound.
+// Lanes per row: the smallest power of two with at most ~4 entries per lane, so a
+// lane has several independent (value, column, gather) loads in flight (one entry
+// per lane is latency-bound). A sweep too small to fill the GPU (fewer than 2^18
+// lanes) is bound by each row's serial chain instead of by throughput: it takes
+// twice the lanes (~2 entries per lane). Measured at 1M cells: level-1/2 sweeps
+// 21.9/23.6 us at ~4 per lane against 25.5/28.4 us at ~2; level-4+ sweeps and
+// coarse restrictions 7-12 us at ~4 against 5-8 us at ~2.
 int csr_group_for(int n_rows, int nnz) {
   if (n_rows <= 0) return 1;
   const double mean = (double)nnz / (double)n_rows;
   int g = 1;
   while (g < 64 && g * 4 < mean) g <<= 1;
+  if ((long long)n_rows * g < (1LL << 18) && g < 64 && g * 2 < mean) g <<= 1;
   return g;
 }
 
