@@ -35,7 +35,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-dims", default="50x50x40")
     ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--kernel", default="st7_jacobi_f32", help="kernel timed for the roofline figure")
+    ap.add_argument("--kernels", default="csr_jacobi_f32,st7_jacobi_f32",
+                    help="kernels timed for the roofline figures: the first (the dominant one, the L1-Jacobi "
+                         "sweep of the CSR levels) is `roofline`, the others `roofline_other`")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing of the history-match case")
     ap.add_argument("--no-rl", action="store_true", help="skip the batched reinforcement-learning environment leg")
     ap.add_argument("--rl-envs", type=int, default=64, help="copies stepped together by the batched environment")
@@ -53,10 +55,11 @@ def make_sim(dims, precision):
     return cfg, build(cfg, fields, "cuda")
 
 
-def run(dims, precision, steps, warmup, kernel_steps=0, kernel="st7_jacobi_f32"):
-    """warmup + `steps` timed steps (no instrumentation), then `kernel_steps` more
-    steps with every launch of `kernel` bracketed by HIP events on its stream
-    (V-cycle graph replay is bypassed while a kernel is timed)."""
+def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
+    """warmup + `steps` timed steps (no instrumentation), then for each name in
+    `kernels`, `kernel_steps` more steps with every launch of that kernel
+    bracketed by HIP events on its stream (V-cycle graph replay is bypassed while
+    a kernel is timed). Returns the kernel totals by name."""
     import torch
     import impes_hip as H
 
@@ -74,16 +77,16 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernel="st7_jacobi_f32")
         its.append(int(sim.last_solve.get("iterations", 0)))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    kt = None
-    if kernel_steps:
+    kts = {}
+    for kernel in (kernels if kernel_steps else ()):
         H.ktimer_reset(enable=True, kernel=kernel)
         for _ in range(kernel_steps):
             if not sim.run_step(dt):
                 raise RuntimeError("kernel-timing step failed")
         torch.cuda.synchronize()
-        kt = H.ktimer_read()
+        kts[kernel] = H.ktimer_read()
         H.ktimer_reset(enable=False)
-    return sim, el, its, kt
+    return sim, el, its, kts
 
 
 def cpu_baseline(dims, steps):
@@ -301,23 +304,32 @@ def main():
                  "fgmres_iterations_fp64": its64}
         del sim64
 
-    # roofline of the dominant kernel, on a fresh run of the same workload
-    sim_k, _, _, kt = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernel=a.kernel)
+    # rooflines of the dominant kernel and of the others named, on a fresh run of
+    # the same workload
+    names = [k for k in a.kernels.split(",") if k]
+    sim_k, _, _, kts = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernels=names)
     levels = sim_k.amg_levels()
     del sim_k
-    roof = None
-    if kt is not None and kt["launches"] > 0:
+    tf = os.path.join(ROOT, "profiles", "hbm_traffic_1M_gas_injection.json")
+    traffic_table = json.load(open(tf)) if os.path.exists(tf) else {}
+
+    def roofline_of(kt):
+        if kt is None or kt["launches"] <= 0:
+            return None
+        # a kernel that runs on several levels: total algorithmic bytes over total
+        # launch time (bytes and time per launch are both averages over the levels)
         per_launch_s = kt["ms"] * 1e-3 / kt["launches"]
         bytes_per_launch = kt["bytes"] / kt["launches"]
         achieved = bytes_per_launch / per_launch_s / 1e9
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "hbm_traffic_1M_gas_injection.json")
-        if os.path.exists(tf):
-            traffic = json.load(open(tf)).get(kt["name"], {}).get("hbm_bytes_per_launch")
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kt["name"],
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic_table.get(kt["name"], {}).get("hbm_bytes_per_launch"), "kernel": kt["name"],
                 "launches": kt["launches"], "avg_us": per_launch_s * 1e6,
                 "algorithmic_bytes_per_launch": bytes_per_launch}
+
+    roof = roofline_of(kts.get(names[0])) if names else None
+    if len(names) > 1:
+        extra["roofline_other"] = {k: roofline_of(kts.get(k)) for k in names[1:]}
 
     if not a.no_adjoint:
         extra["adjoint"] = adjoint_timing()
