@@ -240,6 +240,46 @@ def decisions_changed(a: list, b: list) -> int:
     return sum(x != y for x, y in zip(a, b)) + abs(len(a) - len(b))
 
 
+class BranchLog(list):
+    """A branch record (the list `_record` fills) that also keeps the decisions
+    themselves, in order: upwind masks, the bhp_min switch, choke signs and picks,
+    the clamp of the choke factor, rate limits, the substep count N and the four
+    saturation clamp masks of every substep. A run given ``BranchLog(replay=log)``
+    takes log's decisions at those sites instead of evaluating them, i.e. it
+    evaluates the smooth branch that contains log's run: central differences of
+    two such runs around the recorded point then measure the derivative the
+    adjoint computes. Kinks not replayed: the PVT table segments, the relperm and
+    capillary clamps, and the 1e-12 denominator floors."""
+
+    def __init__(self, replay: "BranchLog | None" = None):
+        super().__init__()
+        self.kept = []
+        self.replay = replay
+        self._next = 0
+
+    def take(self, value):
+        if self.replay is not None:
+            value = self.replay.kept[self._next]
+            self._next += 1
+            return value
+        self.kept.append(value.detach().clone() if torch.is_tensor(value) else value)
+        return value
+
+
+def _decide(branches, value, tag=None):
+    """A piecewise decision (bool mask tensor, bool or int): kept or replayed by a
+    BranchLog, and recorded in the branch record (masks by `_record`, the others
+    as (tag, value))."""
+    if isinstance(branches, BranchLog):
+        value = branches.take(value)
+    if branches is not None:
+        if torch.is_tensor(value):
+            _record(branches, value)
+        else:
+            branches.append((tag, value))
+    return value
+
+
 # ---------------------------------------------------------------- stencil helpers
 def _px(t, lo, hi):
     return Fn.pad(t, (0, 0, 0, 0, lo, hi))
@@ -305,9 +345,7 @@ def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg, branches=None):
     mob_w, mob_o, mob_g = krw / mu_w, kro / mu_o, krg / mu_g
     mob_t = mob_w + mob_o + mob_g
     dpx, dpy, dpz = p[:-1] - p[1:], p[:, :-1] - p[:, 1:], p[:, :, :-1] - p[:, :, 1:]
-    ux, uy, uz = (dpx > 0).detach(), (dpy > 0).detach(), (dpz > 0).detach()
-    for m in (ux, uy, uz):
-        _record(branches, m)
+    ux, uy, uz = (_decide(branches, (d > 0).detach()) for d in (dpx, dpy, dpz))
     xl, xh, yl, yh, zl, zh = _faces(mob_t)
     tx, ty, tz = Tx * torch.where(ux, xl, xh), Ty * torch.where(uy, yl, yh), Tz * torch.where(uz, zl, zh)
     n = p.numel()
@@ -324,9 +362,7 @@ def pressure_system(case: ThreePhaseCase, T, values, p, sw, sg, branches=None):
                 denom = w["wi"] * mt[c] + 1e-12
                 preq = pp[c] - rate / denom
                 pmin = float(w["bhp_min"]) * 1e6
-                if float(preq.detach()) < pmin:
-                    if branches is not None:
-                        branches.append(("bhp_min", wn))
+                if _decide(branches, float(preq.detach()) < pmin, ("bhp_min", wn)):
                     term = w["wi"] * mt[c]
                     terms.append(term)
                     bhp_diag = bhp_diag.index_add(0, idx, term.reshape(1))
@@ -372,11 +408,18 @@ def _mx(a):
 
 def _choke(alpha, up, dn, dS, branches=None):
     """min(alpha, up / max(dS, 1e-12)) when dS > 0, else min(alpha, dn / max(|dS|, 1e-12))."""
-    pos = float(dS.detach()) > 0.0
+    pos = _decide(branches, float(dS.detach()) > 0.0, "choke_sign")
     cand = up / _mx(dS) if pos else dn / _mx(torch.abs(dS))
-    if branches is not None:
-        branches.append(("choke", pos, float(cand.detach()) < float(alpha.detach())))
-    return torch.minimum(alpha, cand)
+    if _decide(branches, float(cand.detach()) < float(alpha.detach()), "choke_pick"):
+        return cand
+    return alpha
+
+
+def _clamp01(alpha, branches=None):
+    """clamp(alpha, 0, 1) of a choke factor (alpha <= 1 by construction)."""
+    if _decide(branches, float(alpha.detach()) < 0.0, "choke_floor"):
+        return torch.zeros_like(alpha)
+    return alpha
 
 
 def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_old, ps, branches=None):
@@ -389,9 +432,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
     mob_w, mob_o, mob_g = krw / mu_w, kro / mu_o, krg / mu_g
     mob_t = mob_w + mob_o + mob_g
     dpx, dpy, dpz = P[:-1] - P[1:], P[:, :-1] - P[:, 1:], P[:, :, :-1] - P[:, :, 1:]
-    ux, uy, uz = (dpx > 0).detach(), (dpy > 0).detach(), (dpz > 0).detach()
-    for m in (ux, uy, uz):
-        _record(branches, m)
+    ux, uy, uz = (_decide(branches, (d > 0).detach()) for d in (dpx, dpy, dpz))
     wl = _faces(mob_w)
     gl = _faces(mob_g)
     mwx, mwy, mwz = torch.where(ux, wl[0], wl[1]), torch.where(uy, wl[2], wl[3]), torch.where(uz, wl[4], wl[5])
@@ -447,7 +488,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                     up, dn = torch.clamp((1.0 - so_r - sg) - sw, min=0.0), sw - sw_cr
                 dS = (case.dt / _mx(pvf[c])) * q_total
                 alpha = _choke(one, up, dn, dS, branches)
-                q_total = q_total * torch.clamp(alpha, 0.0, 1.0)
+                q_total = q_total * _clamp01(alpha, branches)
                 if w["gas"]:
                     q_g = add(q_g, q_total)
                 else:
@@ -473,7 +514,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                     dSw = (case.dt / _mx(pvf[c])) * (q_total * fw_)
                     dSg = (case.dt / _mx(pvf[c])) * (q_total * fg_)
                     alpha = _choke(_choke(one, sw_up, sw_dn, dSw, branches), sg_up, sg_dn, dSg, branches)
-                    q_total = q_total * torch.clamp(alpha, 0.0, 1.0)
+                    q_total = q_total * _clamp01(alpha, branches)
                     lim = w["limits"]
                     if lim:
                         d_o = fo_ / _mx(bo) + fg_ * rv / _mx(bg)
@@ -488,9 +529,8 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                             m = cand[0]
                             for cnd in cand[1:]:
                                 m = torch.minimum(m, cnd)
-                            limited = float(torch.abs(q_total).detach()) > float(m.detach())
-                            if branches is not None:
-                                branches.append(("limit", wn, limited))
+                            limited = _decide(branches, float(torch.abs(q_total).detach()) > float(m.detach()),
+                                              ("limit", wn))
                             if limited:
                                 q_total = -m
                 q_w = add(q_w, q_total * fw_)
@@ -523,17 +563,23 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
         rmax = float(rat.max())
         rmax = rmax if math.isfinite(rmax) else 0.0
         N = min(max(int(math.ceil(rmax)), 1), case.max_substeps)
+    N = _decide(branches, N, "substeps")
     sw, sg = sw_old, sg_old
     for _ in range(N):
+        # sw = min(max(sw_u, sw_cr), 1 - so_r - sg_u), then sg likewise against the
+        # new sw, with the active clamps as (replayable) decisions
         sw_u = sw + dSw / N
         sg_u = sg + dSg / N
-        sw = torch.minimum(torch.clamp(sw_u, min=sw_cr), 1.0 - so_r - sg_u)
-        sg = torch.minimum(torch.clamp(sg_u, min=sg_cr), 1.0 - so_r - sw)
-        if branches is not None:  # active clamps of this substep
-            _record(branches, sw_u < sw_cr)
-            _record(branches, torch.clamp(sw_u, min=sw_cr) > 1.0 - so_r - sg_u)
-            _record(branches, sg_u < sg_cr)
-            _record(branches, torch.clamp(sg_u, min=sg_cr) > 1.0 - so_r - sw)
+        lo = _decide(branches, (sw_u < sw_cr).detach())
+        a = torch.where(lo, torch.full_like(sw_u, sw_cr), sw_u)
+        cap = 1.0 - so_r - sg_u
+        hi = _decide(branches, (a > cap).detach())
+        sw = torch.where(hi, cap, a)
+        lo = _decide(branches, (sg_u < sg_cr).detach())
+        a = torch.where(lo, torch.full_like(sg_u, sg_cr), sg_u)
+        cap = 1.0 - so_r - sw
+        hi = _decide(branches, (a > cap).detach())
+        sg = torch.where(hi, cap, a)
     return sw, sg, N
 
 
