@@ -77,7 +77,9 @@ __global__ void k_props_pressure(GridDev G, FluidParams F, PvtDev T, double dt, 
     const double pp = p[c], s_w = sw[c], s_g = sg[c];
     double kro, krw, krg, muo, muw, mug;
     relperms(F, s_w, s_g, kro, krw, krg);
-    viscosities(F, T, pp, muo, muw, mug);
+    PvtAt at{};
+    if (T.n > 0) at = pvt_locate(T, pp);
+    viscosities(F, T, pp, muo, muw, mug, &at);
     const double a = krw / muw, o = kro / muo, g = krg / mug;
     mw[c] = a;
     mo[c] = o;
@@ -85,13 +87,13 @@ __global__ void k_props_pressure(GridDev G, FluidParams F, PvtDev T, double dt, 
     mt[c] = a + o + g;
     if (G.gravity) {
       double r1, r2, r3;
-      densities(F, T, pp, r1, r2, r3);
+      densities(F, T, pp, r1, r2, r3, &at);
       rw[c] = r1;
       ro[c] = r2;
       rg[c] = r3;
     }
     if (F.pc_scale > 0.0) pc[c] = pc_ow(F, s_w);
-    acc[c] = pv[c] * total_compressibility(F, T, pp, s_w, s_g) / dt;
+    acc[c] = pv[c] * total_compressibility(F, T, pp, s_w, s_g, &at) / dt;
   }
 }
 
@@ -224,13 +226,15 @@ __global__ void k_props_sat(GridDev G, FluidParams F, PvtDev T, const double* __
     const double pp = p[c];
     double kro, krw, krg, muo, muw, mug;
     relperms(F, sw[c], sg[c], kro, krw, krg);
-    viscosities(F, T, pp, muo, muw, mug);
+    PvtAt at{};
+    if (T.n > 0) at = pvt_locate(T, pp);
+    viscosities(F, T, pp, muo, muw, mug, &at);
     mw[c] = krw / muw;
     mo[c] = kro / muo;
     mg[c] = krg / mug;
     if (G.gravity) {
       double r1, r2, r3;
-      densities(F, T, pp, r1, r2, r3);
+      densities(F, T, pp, r1, r2, r3, &at);
       rw[c] = r1;
       rg[c] = r3;
     }

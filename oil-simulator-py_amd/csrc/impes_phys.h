@@ -48,20 +48,76 @@ struct WellDesc {
 };
 
 // np.interp(p/1e6, P, col, left=col[0], right=col[-1]); products and sums are
-// rounded separately, as numpy does.
-__device__ __forceinline__ double pvt_eval(const PvtDev& T, int key, double p_pa) {
-  const double x = p_pa / 1e6;
+// rounded separately, as numpy does. The table segment depends only on the
+// pressure, so it is located once (pvt_locate) and read for every column that
+// pressure needs (pvt_at): one binary search per pressure instead of one per
+// property.
+struct PvtAt {
+  double x;  // p / 1e6
+  int lo;    // segment (inside the table)
+  int side;  // 0 inside, -1 below the first node, 1 at or above the last, 2 NaN
+};
+
+__device__ __forceinline__ PvtAt pvt_locate(const PvtDev& T, double p_pa) {
+  PvtAt a;
+  a.x = p_pa / 1e6;
+  a.lo = 0;
   const double* xp = T.P;
-  const double* fp = T.col + key * T.n;
   const int n = T.n;
-  if (!(x >= xp[0])) return (x != x) ? x : fp[0];
-  if (x >= xp[n - 1]) return fp[n - 1];
+  if (!(a.x >= xp[0])) {
+    a.side = (a.x != a.x) ? 2 : -1;
+    return a;
+  }
+  if (a.x >= xp[n - 1]) {
+    a.side = 1;
+    return a;
+  }
   int lo = 0, hi = n - 1;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (xp[mid] <= x) lo = mid; else hi = mid;
+    if (xp[mid] <= a.x) lo = mid; else hi = mid;
   }
-  return __dadd_rn(__dmul_rn(T.slope[key * (n - 1) + lo], __dsub_rn(x, xp[lo])), fp[lo]);
+  a.lo = lo;
+  a.side = 0;
+  return a;
+}
+
+// The same location found by walking from a nearby located pressure (a few table
+// nodes away): the segment with xp[lo] <= x < xp[lo + 1] is unique in the
+// increasing node list, so the walk ends where the binary search does.
+__device__ __forceinline__ PvtAt pvt_locate_near(const PvtDev& T, double p_pa, const PvtAt& near) {
+  if (near.side != 0) return pvt_locate(T, p_pa);
+  PvtAt a;
+  a.x = p_pa / 1e6;
+  a.lo = 0;
+  const double* xp = T.P;
+  const int n = T.n;
+  if (!(a.x >= xp[0])) {
+    a.side = (a.x != a.x) ? 2 : -1;
+    return a;
+  }
+  if (a.x >= xp[n - 1]) {
+    a.side = 1;
+    return a;
+  }
+  int lo = near.lo;
+  while (lo > 0 && xp[lo] > a.x) --lo;
+  while (lo < n - 2 && xp[lo + 1] <= a.x) ++lo;
+  a.lo = lo;
+  a.side = 0;
+  return a;
+}
+
+__device__ __forceinline__ double pvt_at(const PvtDev& T, int key, const PvtAt& a) {
+  const double* fp = T.col + key * T.n;
+  if (a.side == 2) return a.x;
+  if (a.side == -1) return fp[0];
+  if (a.side == 1) return fp[T.n - 1];
+  return __dadd_rn(__dmul_rn(T.slope[key * (T.n - 1) + a.lo], __dsub_rn(a.x, T.P[a.lo])), fp[a.lo]);
+}
+
+__device__ __forceinline__ double pvt_eval(const PvtDev& T, int key, double p_pa) {
+  return pvt_at(T, key, pvt_locate(T, p_pa));
 }
 
 __device__ __forceinline__ double clamp01(double x) { return fmin(fmax(x, 0.0), 1.0); }
@@ -102,12 +158,14 @@ __device__ __forceinline__ double pc_og(const FluidParams& F, double sg) {
 }
 
 // phase viscosities in Pa s (simulation.py:875-877)
+// `at`: the located pressure p (pvt_locate(T, p)) when the caller has it
 __device__ __forceinline__ void viscosities(const FluidParams& F, const PvtDev& T, double p, double& mo, double& mw,
-                                            double& mg) {
+                                            double& mg, const PvtAt* at = nullptr) {
   if (T.n > 0) {
-    mo = pvt_eval(T, kMuO, p) * 1e-3;
-    mw = pvt_eval(T, kMuW, p) * 1e-3;
-    mg = pvt_eval(T, kMuG, p) * 1e-3;
+    const PvtAt a = at ? *at : pvt_locate(T, p);
+    mo = pvt_at(T, kMuO, a) * 1e-3;
+    mw = pvt_at(T, kMuW, a) * 1e-3;
+    mg = pvt_at(T, kMuG, a) * 1e-3;
   } else {
     mo = F.mu_o;
     mw = F.mu_w;
@@ -116,16 +174,20 @@ __device__ __forceinline__ void viscosities(const FluidParams& F, const PvtDev& 
 }
 
 // fluid.py:190-222 (central difference of B over +-0.1 MPa); fluid.py:47-48 without PVT
+// `at`: the located pressure p when the caller has it
 __device__ __forceinline__ double total_compressibility(const FluidParams& F, const PvtDev& T, double p, double sw,
-                                                        double sg) {
+                                                        double sg, const PvtAt* at = nullptr) {
   if (T.n == 0) return F.cf;
   const double pm = p / 1e6;
   double c[3];
   const int keys[3] = {kBw, kBo, kBg};
+  const double p0 = pm * 1e6;
+  const PvtAt a0 = (at && p0 == p) ? *at : pvt_locate(T, p0);
+  const PvtAt ap = pvt_locate_near(T, (pm + 0.1) * 1e6, a0), am = pvt_locate_near(T, (pm - 0.1) * 1e6, a0);
   for (int q = 0; q < 3; ++q) {
-    const double b0 = pvt_eval(T, keys[q], pm * 1e6);
-    const double bp = pvt_eval(T, keys[q], (pm + 0.1) * 1e6);
-    const double bm = pvt_eval(T, keys[q], (pm - 0.1) * 1e6);
+    const double b0 = pvt_at(T, keys[q], a0);
+    const double bp = pvt_at(T, keys[q], ap);
+    const double bm = pvt_at(T, keys[q], am);
     const double dB = (bp - bm) / (2 * 0.1);
     c[q] = -(dB / (b0 + 1e-12)) / 1e6;
   }
@@ -135,15 +197,16 @@ __device__ __forceinline__ double total_compressibility(const FluidParams& F, co
 
 // fluid.py:129-158, 330-375
 __device__ __forceinline__ void densities(const FluidParams& F, const PvtDev& T, double p, double& rw, double& ro,
-                                          double& rg) {
+                                          double& rg, const PvtAt* at = nullptr) {
   if (T.n == 0) {
     rw = F.rho_w_ref * (1.0 + F.c_w * (p - 1e5));
     ro = F.rho_o_ref * (1.0 + F.c_o * (p - 1e5));
     rg = F.rho_g_ref * (1.0 + F.c_g * (p - 1e5));
     return;
   }
-  const double bw = pvt_eval(T, kBw, p), bo = pvt_eval(T, kBo, p), rs = pvt_eval(T, kRs, p);
-  const double bg = pvt_eval(T, kBg, p), rv = pvt_eval(T, kRv, p);
+  const PvtAt a = at ? *at : pvt_locate(T, p);
+  const double bw = pvt_at(T, kBw, a), bo = pvt_at(T, kBo, a), rs = pvt_at(T, kRs, a);
+  const double bg = pvt_at(T, kBg, a), rv = pvt_at(T, kRv, a);
   rw = F.rho_w_ref / (bw + 1e-12);
   ro = (F.rho_o_ref + rs * F.rho_g_ref) / (bo + 1e-12);
   rg = (F.rho_g_ref + rv * F.rho_o_ref) / (bg + 1e-12);
