@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
     R[kWBal] = NAN;
     const int c = d.cell;
     const double P = X.p[c];
+    const PvtAt atP = pvt ? pvt_locate(X.T, P) : PvtAt{};  // every lookup below is at P
     const double mwc = X.mw[c], moc = X.mo[c], mgc = X.mg[c];
     const double mtc = mwc + moc + mgc;
     const double fw = mwc / (mtc + 1e-10), fg = mgc / (mtc + 1e-10), fo = moc / (mtc + 1e-10);
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
     if (d.control == 0) {
       const double q_base = (d.value * F.rate_scale) / kDay;
       double q;
-      if (!d.producer && d.surface && pvt) q = q_base * pvt_eval(X.T, d.gas ? kBg : kBw, P);
+      if (!d.producer && d.surface && pvt) q = q_base * pvt_at(X.T, d.gas ? kBg : kBw, atP);
       else q = q_base;
       q *= d.producer ? -1.0 : 1.0;
       if (!d.producer) {
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
           dqg += q;
           sum_qg += q;
           if (pvt) {
-            const double bg = pvt_eval(X.T, kBg, P), rv = pvt_eval(X.T, kRv, P);
+            const double bg = pvt_at(X.T, kBg, atP), rv = pvt_at(X.T, kRv, atP);
             const double qs = fmax(q, 0.0) / mx12(bg) * kDay;
             gas_in += qs * (dt / kDay);
             oil_in += rv * qs * (dt / kDay);
@@ -391,8 +392,8 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
       } else {
         const bool surface = d.surface && pvt;
         if (surface || pvt) {
-          const double bo = pvt_eval(X.T, kBo, P), bw = pvt_eval(X.T, kBw, P), bg = pvt_eval(X.T, kBg, P);
-          const double rs = pvt_eval(X.T, kRs, P), rv = pvt_eval(X.T, kRv, P);
+          const double bo = pvt_at(X.T, kBo, atP), bw = pvt_at(X.T, kBw, atP), bg = pvt_at(X.T, kBg, atP);
+          const double rs = pvt_at(X.T, kRs, atP), rv = pvt_at(X.T, kRv, atP);
           if (surface) {
             if (d.surface_phase == 0) q = -(q_base / mx12(fo / mx12(bo) + fg * rv / mx12(bg)));
             else if (d.surface_phase == 1) q = -(q_base * bw) / fmax(fw, 1e-8);
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
           dqg += -q;
           sum_qg += -q;
           if (pvt) {
-            const double bg = pvt_eval(X.T, kBg, P), rv = pvt_eval(X.T, kRv, P);
+            const double bg = pvt_at(X.T, kBg, atP), rv = pvt_at(X.T, kRv, atP);
             const double qs = fmax(-q, 0.0) / mx12(bg) * kDay;
             gas_in += qs * (dt / kDay);
             oil_in += rv * qs * (dt / kDay);
@@ -478,8 +479,8 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
         sum_qw += -q * fw;
         sum_qg += -q * fg;
         if (pvt) {
-          const double bo = pvt_eval(X.T, kBo, P), bg = pvt_eval(X.T, kBg, P);
-          const double rs = pvt_eval(X.T, kRs, P), rv = pvt_eval(X.T, kRv, P);
+          const double bo = pvt_at(X.T, kBo, atP), bg = pvt_at(X.T, kBg, atP);
+          const double rs = pvt_at(X.T, kRs, atP), rv = pvt_at(X.T, kRv, atP);
           const double qa = fabs(q);
           const double qo_res = qa * fo, qg_res = qa * fg;
           oil_out += dt * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
@@ -710,9 +711,10 @@ __global__ __launch_bounds__(256) void k_substep_ledger(SatWellCtx X, const doub
     const double* R = rep + kReportHead + kWellSlots * w;
     const int c = d.cell;
     const double P = X.p[c];
-    const double muw = pvt_eval(X.T, kMuW, P), muo = pvt_eval(X.T, kMuO, P), mug = pvt_eval(X.T, kMuG, P);
-    const double bo = pvt_eval(X.T, kBo, P), bw = pvt_eval(X.T, kBw, P), bg = pvt_eval(X.T, kBg, P);
-    const double rs = pvt_eval(X.T, kRs, P), rv = pvt_eval(X.T, kRv, P);
+    const PvtAt atP = pvt_locate(X.T, P);  // every lookup below is at P
+    const double muw = pvt_at(X.T, kMuW, atP), muo = pvt_at(X.T, kMuO, atP), mug = pvt_at(X.T, kMuG, atP);
+    const double bo = pvt_at(X.T, kBo, atP), bw = pvt_at(X.T, kBw, atP), bg = pvt_at(X.T, kBg, atP);
+    const double rs = pvt_at(X.T, kRs, atP), rv = pvt_at(X.T, kRv, atP);
     const double a = dsw[c] / N, b = dsg[c] / N;
     double s_w = X.sw[c], s_g = X.sg[c];
     for (int q = 0; q <= sub; ++q) {
