@@ -125,6 +125,13 @@ int impes_amg_export_scaling(const impes_amg* amg, double* dhalf_inv, void* stre
  * (32: b and x are FP64 buffers, converted at the boundary).
  * Replaces ClassicalAMG._v_cycle, src/solver/classical_amg.py:1037-1106. */
 int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero, void* stream);
+/* The transposed cycle, V(post,pre): for the symmetric IMPES operator (R = P^T,
+ * diagonal L1-Jacobi splitting, symmetric coarse solve) the zero-start cycle
+ * satisfies V(pre,post)^T = V(post,pre). No counterpart in the reference; it is
+ * the transposed preconditioner of an adjoint solve (the reference's V(3,2)
+ * cycle is not self-adjoint; the default V(2,2) is). */
+int impes_amg_vcycle_transposed(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero,
+                                void* stream);
 /* Preconditioner of the row-scaled system D^-1 A: z = S V(S D v), S = D^-1/2 or I.
  * Replaces the apply closure of amg_preconditioner, src/solver/pressure_amg.py:155-158. */
 int impes_amg_precondition(impes_amg* amg, int precision, const double* v, double* z, void* stream);

@@ -112,6 +112,11 @@ class Hierarchy {
   // One V-cycle in the equilibrated space: x <- V(b) (x_zero: start from 0).
   void vcycle64(const double* b, double* x, bool x_zero, hipStream_t s);
   void vcycle32(const float* b, float* x, bool x_zero, hipStream_t s);
+  // The cycle with the pre- and post-smoothing counts swapped. For a symmetric
+  // operator (R = P^T, the diagonal L1-Jacobi splitting, a symmetric coarse
+  // solve) the zero-start V(nu1, nu2) cycle B satisfies B^T = V(nu2, nu1): this is
+  // the transposed preconditioner (identical to the cycle when nu1 = nu2).
+  void set_transposed(bool on) { transposed_ = on; }
 
   // Preconditioner of the row-scaled system D^-1 A x = D^-1 b:
   // z = S .* V(S .* D .* v)  with S = D^-1/2 (equilibrated) or I.
@@ -174,6 +179,7 @@ class Hierarchy {
   void capture_vcycle(int precision);
 
   bool graphs_ = true;
+  bool transposed_ = false;  // eager cycles of vcycle64/32 with nu1 and nu2 swapped
   hipStream_t cap_stream_ = nullptr;       // capture only; replays run on the caller's stream
   hipGraphExec_t vexec_[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64 V-cycle
 

@@ -785,13 +785,25 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
   if (cur != x) Sel<T>::copy(x, cur, L.n, s);
 }
 
+namespace {
+// swaps the smoothing counts for one eager cycle (restored on every exit path)
+struct SmoothSwap {
+  AmgParams& p;
+  bool on;
+  SmoothSwap(AmgParams& prm, bool swap) : p(prm), on(swap) { if (on) std::swap(p.pre_smooth, p.post_smooth); }
+  ~SmoothSwap() { if (on) std::swap(p.pre_smooth, p.post_smooth); }
+};
+}  // namespace
+
 void Hierarchy::vcycle64(const double* b, double* x, bool x_zero, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "vcycle before setup");
+  SmoothSwap sw(prm_, transposed_);
   vcycle_level<double>(0, b, x, x_zero, s);
 }
 
 void Hierarchy::vcycle32(const float* b, float* x, bool x_zero, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "vcycle before setup");
+  SmoothSwap sw(prm_, transposed_);
   vcycle_level<float>(0, b, x, x_zero, s);
 }
 

@@ -288,9 +288,15 @@ int impes_amg_export_scaling(const impes_amg* amg, double* dhalf_inv, void* stre
   });
 }
 
-int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero, void* stream) {
-  return guarded(__func__, [&] {
+static int vcycle_entry(const char* fn, impes_amg* amg, int precision, const double* b, double* x, int x_is_zero,
+                        bool transposed, void* stream) {
+  return guarded(fn, [&] {
     IMPES_REQUIRE(amg && b && x, "null pointer");
+    struct Restore {
+      impes::Hierarchy& h;
+      ~Restore() { h.set_transposed(false); }
+    } restore{amg->h};
+    amg->h.set_transposed(transposed);
     IMPES_REQUIRE(amg->h.num_levels() > 0, "not set up");
     IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
     hipStream_t s = as_stream(stream);
@@ -312,6 +318,15 @@ int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, 
       IMPES_HIP_CHECK(hipStreamSynchronize(s));
     }
   });
+}
+
+int impes_amg_vcycle(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero, void* stream) {
+  return vcycle_entry(__func__, amg, precision, b, x, x_is_zero, false, stream);
+}
+
+int impes_amg_vcycle_transposed(impes_amg* amg, int precision, const double* b, double* x, int x_is_zero,
+                                void* stream) {
+  return vcycle_entry(__func__, amg, precision, b, x, x_is_zero, true, stream);
 }
 
 int impes_amg_set_graphs(impes_amg* amg, int enable) {

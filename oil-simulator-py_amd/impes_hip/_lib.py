@@ -123,6 +123,7 @@ SIGNATURES = {
     "impes_amg_export_level": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "impes_amg_export_scaling": [_P, _P, _P],
     "impes_amg_vcycle": [_P, _I, _P, _P, _I, _P],
+    "impes_amg_vcycle_transposed": [_P, _I, _P, _P, _I, _P],
     "impes_amg_precondition": [_P, _I, _P, _P, _P],
     "impes_amg_set_graphs": [_P, _I],
     "impes_solver_create": [C.POINTER(_P)],

@@ -215,12 +215,14 @@ class AmgHierarchy:
         call("impes_amg_export_scaling", self._h, ptr(t), stream_ptr())
         return t
 
-    def vcycle(self, b, x=None, precision=64):
-        """x <- V(b) in the equilibrated space (x=None: start from zero)."""
+    def vcycle(self, b, x=None, precision=64, transposed=False):
+        """x <- V(b) in the equilibrated space (x=None: start from zero);
+        transposed: the cycle with pre- and post-smoothing swapped (its transpose)."""
         zero = x is None
         if zero:
             x = torch.empty_like(b)
-        call("impes_amg_vcycle", self._h, int(precision), ptr(b), ptr(x), 1 if zero else 0, stream_ptr())
+        fn = "impes_amg_vcycle_transposed" if transposed else "impes_amg_vcycle"
+        call(fn, self._h, int(precision), ptr(b), ptr(x), 1 if zero else 0, stream_ptr())
         return x
 
     def set_graphs(self, enable: bool):

@@ -214,6 +214,35 @@ def test_stencil_update_equals_csr_update(dev, mode, coarsest):
     assert rel(zs.cpu().numpy(), zc.cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize("mode", ["reference", "default"])
+def test_transposed_vcycle_is_the_adjoint(dev, mode):
+    """<x, B y> = <B^T x, y> for the zero-start cycle B and its transposed entry
+    point (smoothing counts swapped), on a symmetric IMPES operator. The
+    reference's V(3,2) is not self-adjoint (the plain cycle fails the identity);
+    the default V(2,2) is, and its transposed cycle is the cycle itself."""
+    import impes_hip as H
+
+    A, *_ = impes_like(16, 12, 6, seed=3)
+    assert abs(A - A.T).max() == 0.0
+    kw = dict(oa.Hierarchy.REFERENCE) if mode == "reference" else {}
+    hg = H.AmgHierarchy(**kw)
+    hg.setup_csr(*to_dev(A, dev))
+    rng = np.random.default_rng(12)
+    x, y = t(rng.standard_normal(A.shape[0]), dev), t(rng.standard_normal(A.shape[0]), dev)
+    By, Bx = hg.vcycle(y), hg.vcycle(x)
+    Btx = hg.vcycle(x, transposed=True)
+    lhs, rhs = float(x @ By), float(Btx @ y)
+    scale = float(x.norm() * By.norm())
+    assert abs(lhs - rhs) <= 1e-12 * scale
+    if mode == "reference":
+        assert abs(lhs - float(Bx @ y)) > 1e-6 * scale
+    else:
+        assert torch.equal(Btx, Bx)
+    # the FP32 cycle satisfies the identity to its own rounding
+    By32, Btx32 = hg.vcycle(y, precision=32), hg.vcycle(x, precision=32, transposed=True)
+    assert abs(float(x @ By32) - float(Btx32 @ y)) <= 1e-5 * scale
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_graph_replay_matches_eager_vcycle(dev, precision):
     """The captured V-cycle graph against eager launches of the same kernels:
