@@ -80,6 +80,7 @@ struct Level {
   DevBuf<float> pv32, rv32;
   // Galerkin product data: AP = A*P (values refreshed on update), next A = R*AP
   int apnnz = 0;
+  int ap_group = 1, ac_group = 1;  // lanes per output entry of the numeric products
   DevBuf<int> aprp, apci;
   DevBuf<double> apv;
   DevBuf<long long> ap_seg, ac_seg;

@@ -522,7 +522,8 @@ void Hierarchy::build_levels(hipStream_t s) {
     // Galerkin product: AP = A*P, Ac = R*AP, both through sorted product lists
     Level nxt;
     auto spgemm = [&](int nrows, const int* lrp, const int* lci, const int* mrp, const int* mci, DevBuf<int>& crp,
-                      DevBuf<int>& cci, int& cnnz, DevBuf<long long>& seg, DevBuf<int>& pe, DevBuf<int>& pf) {
+                      DevBuf<int>& cci, int& cnnz, DevBuf<long long>& seg, DevBuf<int>& pe, DevBuf<int>& pf,
+                      int& group) {
       DevBuf<int> pc, uc;
       DevBuf<long long> pptr;
       DevBuf<unsigned long long> pk;
@@ -549,14 +550,16 @@ void Hierarchy::build_levels(hipStream_t s) {
       pf.alloc(tot);
       launch_spgemm_finalize(nrows, pptr.p, pk.p, crp.p, ex_e.p, ex_f.p, cci.p, seg.p, pe.p, pf.p, tot, cnnz, s);
       IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      group = spgemm_group_for(tot, cnnz);
     };
-    spgemm(n, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf);
+    spgemm(n, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf, A.ap_group);
     A.apv.alloc(A.apnnz);
-    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, s);
+    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, A.ap_group, s);
     nxt.n = nc;
-    spgemm(nc, A.rrp.p, A.rci.p, A.aprp.p, A.apci.p, nxt.rp, nxt.ci, nxt.nnz, A.ac_seg, A.ac_pe, A.ac_pf);
+    spgemm(nc, A.rrp.p, A.rci.p, A.aprp.p, A.apci.p, nxt.rp, nxt.ci, nxt.nnz, A.ac_seg, A.ac_pe, A.ac_pf,
+           A.ac_group);
     nxt.v.alloc(nxt.nnz);
-    launch_spgemm_numeric(nxt.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, nxt.v.p, s);
+    launch_spgemm_numeric(nxt.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, nxt.v.p, A.ac_group, s);
     A.pgroup = csr_group_for(n, A.pnnz);
     A.rgroup = csr_group_for(nc, A.pnnz);
     IMPES_HIP_CHECK(hipStreamSynchronize(s));
@@ -637,8 +640,8 @@ void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
     Level& A = levels_[l];
     if (A.coarsest) break;
     Level& C = levels_[l + 1];
-    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, s);
-    launch_spgemm_numeric(C.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, C.v.p, s);
+    launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, A.ap_group, s);
+    launch_spgemm_numeric(C.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, C.v.p, A.ac_group, s);
   }
   for (int l = 0; l < (int)levels_.size(); ++l) {
     Level& L = levels_[l];

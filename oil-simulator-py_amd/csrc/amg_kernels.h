@@ -77,8 +77,10 @@ void launch_spgemm_finalize(int n, const long long* prod_ptr, const unsigned lon
                             const int* ex_e, const int* ex_f, int* cci, long long* seg, int* pe, int* pf,
                             long long total_products, int nnz_c, hipStream_t s);
 // C.v[o] = sum_{t in seg(o)} Lv[pe[t]] * Mv[pf[t]]
+// lanes per output entry of the numeric product from its number of terms
+int spgemm_group_for(long long terms, int nnz_c);
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
-                           const double* mv, double* cv, hipStream_t s);
+                           const double* mv, double* cv, int group, hipStream_t s);
 
 // coarsest level (coarse.hip). pin is a device flag written by launch_pin_decide and
 // read by the kernels below, so the decision never leaves the device.

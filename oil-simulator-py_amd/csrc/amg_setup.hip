@@ -594,15 +594,28 @@ __global__ void k_spgemm_finalize(int n, const long long* __restrict__ prod_ptr,
   }
 }
 
-__global__ void k_spgemm_numeric(int nnz_c, const long long* __restrict__ seg, const int* __restrict__ pe,
-                                 const int* __restrict__ pf, const double* __restrict__ lv,
-                                 const double* __restrict__ mv, double* __restrict__ cv) {
-  ROW_LOOP(o, nnz_c) {
+// Numeric Galerkin product: output entry o is the sum of lv[pe[t]] * mv[pf[t]]
+// over its product terms t in [seg[o], seg[o+1]). G lanes share an output
+// entry (G from the mean number of terms, as for the CSR sweeps): lane g takes
+// terms g, g+G, ... and the partial sums fold across the G lanes, in a fixed
+// order.
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_spgemm_numeric(int nnz_c, const long long* __restrict__ seg,
+                                                           const int* __restrict__ pe, const int* __restrict__ pf,
+                                                           const double* __restrict__ lv,
+                                                           const double* __restrict__ mv, double* __restrict__ cv) {
+  constexpr int OPB = kBlock / G;
+  const int g = threadIdx.x % G;
+  for (long long base = blockIdx.x * (long long)OPB; base < nnz_c; base += (long long)gridDim.x * OPB) {
+    const long long o = base + threadIdx.x / G;
     double s = 0.0;
-    const long long t1 = seg[o + 1];
+    if (o < nnz_c) {
+      const long long t1 = seg[o + 1];
 #pragma unroll 4
-    for (long long t = seg[o]; t < t1; ++t) s += lv[pe[t]] * mv[pf[t]];
-    cv[o] = s;
+      for (long long t = seg[o] + g; t < t1; t += G) s += lv[pe[t]] * mv[pf[t]];
+    }
+    if (G > 1) s = group_sum<G>(s);
+    if (o < nnz_c && g == 0) cv[o] = s;
   }
 }
 
@@ -717,9 +730,25 @@ void launch_spgemm_finalize(int n, const long long* prod_ptr, const unsigned lon
                             long long total_products, int nnz_c, hipStream_t s) {
   launch_waves(k_spgemm_finalize, n, s, n, prod_ptr, keys, crp, ex_e, ex_f, cci, seg, pe, pf, total_products, nnz_c);
 }
+int spgemm_group_for(long long terms, int nnz_c) {
+  if (nnz_c <= 0) return 1;
+  const double mean = (double)terms / (double)nnz_c;
+  int g = 1;
+  while (g < 8 && g * 4 < mean) g <<= 1;
+  return g;
+}
+
 void launch_spgemm_numeric(int nnz_c, const long long* seg, const int* pe, const int* pf, const double* lv,
-                           const double* mv, double* cv, hipStream_t s) {
-  launch_rows(k_spgemm_numeric, nnz_c, s, nnz_c, seg, pe, pf, lv, mv, cv);
+                           const double* mv, double* cv, int G, hipStream_t s) {
+  if (nnz_c <= 0) return;
+  const int grid = grid_for((long long)nnz_c * G, kBlock, 1 << 16);
+  switch (G) {
+    case 1: k_spgemm_numeric<1><<<grid, kBlock, 0, s>>>(nnz_c, seg, pe, pf, lv, mv, cv); break;
+    case 2: k_spgemm_numeric<2><<<grid, kBlock, 0, s>>>(nnz_c, seg, pe, pf, lv, mv, cv); break;
+    case 4: k_spgemm_numeric<4><<<grid, kBlock, 0, s>>>(nnz_c, seg, pe, pf, lv, mv, cv); break;
+    default: k_spgemm_numeric<8><<<grid, kBlock, 0, s>>>(nnz_c, seg, pe, pf, lv, mv, cv); break;
+  }
+  IMPES_LAUNCH_CHECK();
 }
 
 }  // namespace impes
