@@ -307,3 +307,26 @@ def test_run_writes_the_reference_artifacts(tmp_path):
     mb = sim.mass_balance
     net = sum(mb[ph]["in"] - mb[ph]["out"] for ph in ("water", "gas", "oil"))
     assert f"total net={net:.6e}" in txt
+
+
+def test_update_matrix_rejects_a_size_change():
+    """ClassicalAMG.update_matrix raises ValueError when the new matrix has
+    another size (reference classical_amg.py:1366-1367) and keeps working for
+    a same-size update."""
+    from matrices import impes_like
+    from solver.classical_amg import ClassicalAMG
+
+    def csr(A, dev):
+        return torch.sparse_csr_tensor(torch.tensor(A.indptr), torch.tensor(A.indices), torch.tensor(A.data),
+                                       size=A.shape).to(dev)
+
+    A1, *_ = impes_like(10, 8, 4, seed=2)
+    A2, *_ = impes_like(10, 8, 5, seed=2)
+    amg = ClassicalAMG(csr(A1, "cuda"))
+    with pytest.raises(ValueError):
+        amg.update_matrix(csr(A2, "cuda"))
+    A3, *_ = impes_like(10, 8, 4, seed=2, sigma=1.3)
+    amg.update_matrix(csr(A3, "cuda"))
+    b = torch.ones(A1.shape[0], dtype=torch.float64, device="cuda")
+    x = amg.solve(b, tol=1e-6, max_iter=50)
+    assert torch.isfinite(x).all()
