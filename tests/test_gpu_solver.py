@@ -144,6 +144,33 @@ def test_hierarchy_matches_oracle(dev, case, mode):
     assert rel(xg32, xo) < 1e-4
 
 
+@pytest.mark.parametrize("case", ["max_levels_1", "max_levels_2", "single_level_swept", "single_level_dense"])
+def test_coarsest_level_edge_cases(dev, case):
+    """Coarsest levels above the dense limit (500 rows: solved by 50 L1-Jacobi
+    sweeps, classical_amg.py:1045-1069) after one or two coarsenings, and
+    one-level hierarchies (no pin) solved by sweeps or densely: hierarchy and
+    V-cycle against the oracle."""
+    import impes_hip as H
+
+    if case == "single_level_dense":
+        A, *_ = impes_like(10, 8, 5, seed=4)
+        kw = dict(coarsest_size=1000)
+    else:
+        A, *_ = impes_like(20, 20, 10, seed=4)
+        kw = {"max_levels_1": dict(max_levels=1), "max_levels_2": dict(max_levels=2),
+              "single_level_swept": dict(coarsest_size=5000)}[case]
+    ho = oa.Hierarchy(A, **kw)
+    hg = H.AmgHierarchy(**kw)
+    hg.setup_csr(*to_dev(A, dev))
+    _compare_hierarchy(hg, ho)
+    info = hg.level_info(hg.num_levels - 1)
+    assert bool(info["dense"]) == (case == "single_level_dense")
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    xo = ho.vcycle(b)
+    assert rel(hg.vcycle(t(b, dev), precision=64).cpu().numpy(), xo) < 1e-11
+    assert rel(hg.vcycle(t(b, dev), precision=32).cpu().numpy(), xo) < 1e-4
+
+
 def test_hierarchy_update_values(dev):
     import impes_hip as H
 
