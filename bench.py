@@ -59,15 +59,25 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
     """warmup + `steps` timed steps (no instrumentation), then for each name in
     `kernels`, `kernel_steps` more steps with every launch of that kernel
     bracketed by HIP events on its stream (V-cycle graph replay is bypassed while
-    a kernel is timed). Returns the kernel totals by name."""
+    a kernel is timed). Returns the simulator, the timed seconds, the FGMRES
+    iterations per step, the kernel totals by name and the setup figures: wall
+    time of the first warmup step (it builds the AMG hierarchy) and of the
+    hierarchy build inside it."""
     import torch
     import impes_hip as H
 
     cfg, sim = make_sim(dims, precision)
     dt = cfg["simulation"]["time_step_days"] * 86400.0
-    for _ in range(warmup):
+    setup = {}
+    for w in range(warmup):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         if not sim.run_step(dt):
             raise RuntimeError("warmup step failed")
+        if w == 0:
+            torch.cuda.synchronize()
+            setup = {"first_step_ms": 1e3 * (time.perf_counter() - t0),
+                     "hierarchy_setup_ms": 1e3 * sim._model.amg_setup_seconds()}
     torch.cuda.synchronize()
     its = []
     t0 = time.perf_counter()
@@ -86,7 +96,7 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
         torch.cuda.synchronize()
         kts[kernel] = H.ktimer_read()
         H.ktimer_reset(enable=False)
-    return sim, el, its, kts
+    return sim, el, its, kts, setup
 
 
 def cpu_baseline(dims, steps):
@@ -308,25 +318,25 @@ def main():
     dims = tuple(int(x) for x in a.dims.split("x"))
     ncell = int(np.prod(dims))
 
-    sim32, el32, its32, _ = run(dims, "fp32", a.steps, a.warmup)
+    sim32, el32, its32, _, setup32 = run(dims, "fp32", a.steps, a.warmup)
     value = ncell * a.steps / el32
     p32 = sim32.fluid.pressure.clone()
     del sim32
 
     extra = {}
     if not a.no_fp64:
-        sim64, el64, its64, _ = run(dims, "fp64", a.steps, a.warmup)
+        sim64, el64, its64, _, setup64 = run(dims, "fp64", a.steps, a.warmup)
         p64 = sim64.fluid.pressure
         extra = {"fp64_value": ncell * a.steps / el64, "fp64_ms_per_step": 1e3 * el64 / a.steps,
                  "speedup_fp32_vs_fp64": el64 / el32,
                  "fp32_vs_fp64_pressure_max_rel": float((p32 - p64).abs().max() / p64.abs().max()),
-                 "fgmres_iterations_fp64": its64}
+                 "fgmres_iterations_fp64": its64, "setup_fp64": setup64}
         del sim64
 
     # rooflines of the dominant kernel and of the others named, on a fresh run of
     # the same workload
     names = [k for k in a.kernels.split(",") if k]
-    sim_k, _, _, kts = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernels=names)
+    sim_k, _, _, kts, _ = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernels=names)
     levels = sim_k.amg_levels()
     del sim_k
     tf = os.path.join(ROOT, "profiles", "hbm_traffic_1M_gas_injection.json")
@@ -368,7 +378,7 @@ def main():
                       "dims": list(dims), "cells": ncell, "dt_days": 1.0, "wells": 9, "rel_residual": 1e-8,
                       "amg_levels": [list(x) for x in levels], "fgmres_iterations": its32,
                       "parallelism": "single GPU"},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "setup": setup32}
     out.update(extra)
     print(json.dumps(out), flush=True)
 

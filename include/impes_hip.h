@@ -263,6 +263,10 @@ int impes_model_eval_props(impes_model* m, long long n, const double* p, const d
 int impes_model_eval_pvt(impes_model* m, int key, long long n, const double* p, double* out, void* stream);
 /* Hierarchy summary of the pressure solver (after the first solve). */
 int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_host, int* nnz_host, int max_levels);
+/* Wall time of the last hierarchy build of the pressure solver (0 before the first
+ * solve): the setup part of ClassicalAMG.__init__ (classical_amg.py:699-938), which
+ * the first pressure solve of a run performs and later solves update numerically. */
+int impes_model_amg_setup_seconds(const impes_model* m, double* out);
 
 /* ------------------------------------------------------------------ measurement */
 /* No counterpart in the reference (instrumentation). Times every launch of one

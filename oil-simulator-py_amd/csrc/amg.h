@@ -185,6 +185,9 @@ class Hierarchy {
   hipGraphExec_t vexec_[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64 V-cycle
 
   AmgParams prm_;
+  // device memory of builds: slab-carved small blocks, releases deferred to the
+  // end of the build (declared before the buffers that may hold its blocks)
+  std::shared_ptr<SetupPool> pool_ = std::make_shared<SetupPool>();
   std::vector<Level> levels_;
   bool equil_ = false;
   double scalar_ = 1.0;  // frozen scale of kEquilScalar

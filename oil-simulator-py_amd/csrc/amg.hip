@@ -324,6 +324,11 @@ void Hierarchy::capture_vcycle(int precision) {
 void Hierarchy::setup_impl(int n, int nnz, const int* rp, const int* ci, const double* v, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && nnz >= 1, "amg setup: empty matrix");
   const double t0 = wall();
+  {
+  // every launch and copy of the build goes to s; the scope closes after the
+  // stream synchronisation below (on an error it frees the deferred blocks
+  // with hipFree, which waits for the device)
+  PoolScope scope(pool_);
   drop_graphs();  // the level buffers are about to be rebuilt
   levels_.clear();
   levels_.emplace_back();
@@ -343,6 +348,7 @@ void Hierarchy::setup_impl(int n, int nnz, const int* rp, const int* ci, const d
   if (stencil_) refresh_stencil(s);
   alloc_vectors();
   IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  }
   last_setup_seconds = wall() - t0;
 }
 
@@ -393,7 +399,7 @@ void Hierarchy::equil_factors(const double* diag, bool decide, hipStream_t s) {
     equil_ = small >= (long long)((n - 1) / 2 + 1);
     if (equil_) launch_dhalf_inv(n, diag, dh_.p, s);
     else launch_fill_f64(dh_.p, 1.0, n, s);
-    IMPES_HIP_CHECK(hipStreamSynchronize(s));  // flags freed on return
+    sync_before_release(s);  // flags freed on return
   }
 }
 
@@ -434,8 +440,8 @@ void Hierarchy::build_levels(hipStream_t s) {
     DevBuf<unsigned long long> keys;
     strong.alloc(nnz);
     tpos.alloc(nnz);
-    launch_strength(n, A.rp.p, A.ci.p, A.v.p, prm_.theta, strong.p, s);
-    launch_transpose_pos(n, A.rp.p, A.ci.p, tpos.p, s);
+    launch_strength(n, A.nnz, A.rp.p, A.ci.p, A.v.p, prm_.theta, strong.p, s);
+    launch_transpose_pos(n, A.nnz, A.rp.p, A.ci.p, tpos.p, s);
     {
       // structural symmetry is required by the symmetrised strength graph
       DevBuf<int> neg;
@@ -451,9 +457,9 @@ void Hierarchy::build_levels(hipStream_t s) {
     if (prm_.coarsening == kCoarsenIterated) {
       DevBuf<int> undecided;
       undecided.alloc(n);
-      launch_cf_split_iterated(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, undecided.p, cf0.p, s);
+      launch_cf_split_iterated(n, A.nnz, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, undecided.p, cf0.p, s);
     } else {
-      launch_cf_split(n, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
+      launch_cf_split(n, A.nnz, A.rp.p, A.ci.p, strong.p, tpos.p, deg.p, is_max.p, cf0.p, s);
     }
     if (prm_.dominant > 0.0) launch_force_dominant(n, A.rp.p, A.ci.p, A.v.p, prm_.dominant, cf0.p, s);
     if (prm_.anchor >= 0 && prm_.anchor < n) {
@@ -484,9 +490,9 @@ void Hierarchy::build_levels(hipStream_t s) {
     const long long ncand = read_scalar(cand_ptr.p + n, s);
     keys.alloc(ncand);
     cvals.alloc(ncand);
-    launch_interp_expand(n, A.rp.p, A.ci.p, A.v.p, strong.p, A.cf.p, sum_strong.p, cnt_c.p, f2c.p, cand_ptr.p, keys.p,
+    launch_interp_expand(n, A.nnz, A.rp.p, A.ci.p, A.v.p, strong.p, A.cf.p, sum_strong.p, cnt_c.p, f2c.p, cand_ptr.p, keys.p,
                          cvals.p, s);
-    segmented_sort_u64(keys.p, cand_ptr.p, n, max_segment_length(cand_ptr.p, n, s), s);
+    segmented_sort_u64(keys.p, cand_ptr.p, n, s);
     oc.alloc(ncand);
     ov.alloc(ncand);
     pcnt.alloc(n);
@@ -513,15 +519,15 @@ void Hierarchy::build_levels(hipStream_t s) {
       DevBuf<unsigned long long> tkeys;
       tkeys.alloc(A.pnnz);
       launch_transpose_fill(n, A.prp.p, A.pci.p, A.rrp.p, cursor.p, tkeys.p, s);
-      segmented_sort_u64(tkeys.p, rrp64.p, nc, max_segment_length(rrp64.p, nc, s), s);
+      segmented_sort_u64(tkeys.p, rrp64.p, nc, s);
       A.rci.alloc(A.pnnz);
       A.rv.alloc(A.pnnz);
       launch_transpose_finish(A.pnnz, tkeys.p, A.pv.p, A.rci.p, A.rv.p, s);
-      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      sync_before_release(s);
     }
     // Galerkin product: AP = A*P, Ac = R*AP, both through sorted product lists
     Level nxt;
-    auto spgemm = [&](int nrows, const int* lrp, const int* lci, const int* mrp, const int* mci, DevBuf<int>& crp,
+    auto spgemm = [&](int nrows, long long lnnz, const int* lrp, const int* lci, const int* mrp, const int* mci, DevBuf<int>& crp,
                       DevBuf<int>& cci, int& cnnz, DevBuf<long long>& seg, DevBuf<int>& pe, DevBuf<int>& pf,
                       int& group) {
       DevBuf<int> pc, uc;
@@ -537,8 +543,8 @@ void Hierarchy::build_levels(hipStream_t s) {
       DevBuf<int> ex_e, ex_f;
       ex_e.alloc(tot);
       ex_f.alloc(tot);
-      launch_spgemm_expand(nrows, lrp, lci, mrp, mci, pptr.p, pk.p, ex_e.p, ex_f.p, s);
-      segmented_sort_u64(pk.p, pptr.p, nrows, max_segment_length(pptr.p, nrows, s), s);
+      launch_spgemm_expand(nrows, lnnz, tot, lrp, lci, mrp, mci, pptr.p, pk.p, ex_e.p, ex_f.p, s);
+      segmented_sort_u64(pk.p, pptr.p, nrows, s);
       uc.alloc(nrows);
       launch_spgemm_unique_count(nrows, pptr.p, pk.p, uc.p, s);
       crp.alloc(nrows + 1);
@@ -549,20 +555,21 @@ void Hierarchy::build_levels(hipStream_t s) {
       pe.alloc(tot);
       pf.alloc(tot);
       launch_spgemm_finalize(nrows, pptr.p, pk.p, crp.p, ex_e.p, ex_f.p, cci.p, seg.p, pe.p, pf.p, tot, cnnz, s);
-      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      sync_before_release(s);
       group = spgemm_group_for(tot, cnnz);
     };
-    spgemm(n, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf, A.ap_group);
+    spgemm(n, A.nnz, A.rp.p, A.ci.p, A.prp.p, A.pci.p, A.aprp, A.apci, A.apnnz, A.ap_seg, A.ap_pe, A.ap_pf, A.ap_group);
     A.apv.alloc(A.apnnz);
     launch_spgemm_numeric(A.apnnz, A.ap_seg.p, A.ap_pe.p, A.ap_pf.p, A.v.p, A.pv.p, A.apv.p, A.ap_group, s);
     nxt.n = nc;
-    spgemm(nc, A.rrp.p, A.rci.p, A.aprp.p, A.apci.p, nxt.rp, nxt.ci, nxt.nnz, A.ac_seg, A.ac_pe, A.ac_pf,
+    spgemm(nc, A.pnnz, A.rrp.p, A.rci.p, A.aprp.p, A.apci.p, nxt.rp, nxt.ci, nxt.nnz, A.ac_seg, A.ac_pe, A.ac_pf,
            A.ac_group);
     nxt.v.alloc(nxt.nnz);
     launch_spgemm_numeric(nxt.nnz, A.ac_seg.p, A.ac_pe.p, A.ac_pf.p, A.rv.p, A.apv.p, nxt.v.p, A.ac_group, s);
     A.pgroup = csr_group_for(n, A.pnnz);
     A.rgroup = csr_group_for(nc, A.pnnz);
-    IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    if (trace) IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    else sync_before_release(s);
     if (trace) {
       const double now = wall();
       std::fprintf(stderr, "[amg] level %d n=%d nnz=%d -> nc=%d p_nnz=%d ac_nnz=%d  %.3f s\n", lvl, n, nnz, nc,

@@ -20,18 +20,20 @@ void launch_dhalf_inv(int n, const double* diag, double* dh, hipStream_t s);
 void launch_sym_scale(int n, const int* rp, const int* ci, const double* v, const double* dh, double* vo,
                       hipStream_t s);
 
-void launch_strength(int n, const int* rp, const int* ci, const double* v, double theta, unsigned char* strong,
+void launch_strength(int n, long long nnz, const int* rp, const int* ci, const double* v, double theta,
+                     unsigned char* strong,
                      hipStream_t s);
 // tpos[e] = position of (col,row) in row col, -1 if absent
-void launch_transpose_pos(int n, const int* rp, const int* ci, int* tpos, hipStream_t s);
+void launch_transpose_pos(int n, long long nnz, const int* rp, const int* ci, int* tpos, hipStream_t s);
 // C/F split: symmetrised strong graph, strict local maxima of (deg, hash, index), lift of F points with no C neighbour
-void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos, int* deg,
-                     signed char* is_max, signed char* cf, hipStream_t s);
+void launch_cf_split(int n, long long nnz, const int* rp, const int* ci, const unsigned char* strong,
+                     const int* tpos, int* deg, signed char* is_max, signed char* cf, hipStream_t s);
 // iterated independent-set split on the symmetrised strong graph with the same
 // (deg, hash, index) order: rounds of local-max selection among undecided points,
 // whose undecided neighbours become F, until every point is decided (no lift needed)
-void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos,
-                              int* deg, signed char* newc, int* undecided, signed char* cf, hipStream_t s);
+void launch_cf_split_iterated(int n, long long nnz, const int* rp, const int* ci, const unsigned char* strong,
+                              const int* tpos, int* deg, signed char* newc, int* undecided, signed char* cf,
+                              hipStream_t s);
 // cf[i] = C where |a_ii| > ratio * sum_j!=i |a_ij| (Dirichlet-like rows)
 void launch_force_dominant(int n, const int* rp, const int* ci, const double* v, double ratio, signed char* cf,
                            hipStream_t s);
@@ -44,9 +46,10 @@ void launch_interp_pass_a(int n, const int* rp, const int* ci, const double* v, 
                           const signed char* cf, double* sum_strong, int* cnt_c, hipStream_t s);
 void launch_interp_count(int n, const int* rp, const int* ci, const unsigned char* strong, const signed char* cf,
                          const double* sum_strong, const int* cnt_c, int* cand, hipStream_t s);
-void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
-                          const signed char* cf, const double* sum_strong, const int* cnt_c, const int* f2c,
-                          const long long* cand_ptr, unsigned long long* keys, double* vals, hipStream_t s);
+void launch_interp_expand(int n, long long nnz, const int* rp, const int* ci, const double* v,
+                          const unsigned char* strong, const signed char* cf, const double* sum_strong,
+                          const int* cnt_c, const int* f2c, const long long* cand_ptr, unsigned long long* keys,
+                          double* vals, hipStream_t s);
 // final F-row weights; trunc > 0 / pmax > 0 truncate each F row afterwards: weights below
 // trunc * row max are dropped, then all but the pmax largest (ties: lower coarse column
 // first), and the kept weights are rescaled to the row's previous sum
@@ -67,7 +70,7 @@ void launch_transpose_finish(int nnz, const unsigned long long* keys, const doub
 // SpGEMM C = L*M, symbolic through expanded products sorted per row
 void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, int* cnt, hipStream_t s);
 // keys[t] = (column, local index); ex_e/ex_f[t] = (left entry, right entry) in expansion order
-void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
+void launch_spgemm_expand(int n, long long left_nnz, long long products, const int* lrp, const int* lci, const int* mrp, const int* mci,
                           const long long* prod_ptr, unsigned long long* keys, int* ex_e, int* ex_f, hipStream_t s);
 void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
                                 hipStream_t s);

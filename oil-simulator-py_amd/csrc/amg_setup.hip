@@ -16,6 +16,20 @@ namespace impes {
 
 namespace {
 constexpr int kBlock = 256;
+// Operators with rows longer than this on average (the coarse levels, which have
+// few rows) run the row-walking setup kernels with a wavefront per row; shorter
+// rows use a thread per row. Both forms give identical results.
+constexpr long long kWaveMeanRow = 10;
+inline bool wave_rows(int n, long long nnz) { return nnz > kWaveMeanRow * (long long)n; }
+// The independent-set rounds stop walking a row at its first deciding entry,
+// which favours a thread per row up to longer rows.
+inline bool wave_rows_mis(int n, long long nnz) { return nnz > 24 * (long long)n; }
+// Product expansion: the lanes of a wavefront per row split each right-operand
+// row, so they pay off once those rows are long (products per left entry) or
+// when there are too few rows (< 2^18) to fill the GPU with a thread per row.
+inline bool wave_rows_expand(int n, long long left_nnz, long long products) {
+  return products > 8 * left_nnz || n < (1 << 18);
+}
 
 __device__ __forceinline__ unsigned hash32(unsigned x) {
   x ^= x >> 16;
@@ -35,6 +49,9 @@ __device__ __forceinline__ bool ranks_above(int deg_a, int a, int deg_b, int b) 
 }
 
 #define ROW_LOOP(i, n) for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+#define WAVE_ROW_LOOP(i, n)                                                                  \
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; i < (n); \
+       i += (long long)gridDim.x * blockDim.x / kWave)
 
 // Eight lanes per row: the row sum of |a_ij| folds with shuffles and the first
 // diagonal entry (lowest position) wins, as in the one-thread-per-row definition.
@@ -108,6 +125,56 @@ __global__ void k_strength(int n, const int* __restrict__ rp, const int* __restr
 __global__ void k_transpose_pos(int n, const int* __restrict__ rp, const int* __restrict__ ci, int* __restrict__ tpos) {
   ROW_LOOP(i, n) {
     for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int j = ci[e];
+      int lo = rp[j], hi = rp[j + 1] - 1, pos = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = ci[mid];
+        if (c == (int)i) { pos = mid; break; }
+        if (c < (int)i) lo = mid + 1; else hi = mid - 1;
+      }
+      tpos[e] = pos;
+    }
+  }
+}
+
+// Wavefront-per-row forms of k_strength and k_transpose_pos. The diagonal is the
+// row's first diagonal entry (lowest position over the lanes) and the largest
+// off-diagonal magnitude is a max, so the marks equal the one-thread ones.
+__global__ void k_strength_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ v, double theta, unsigned char* __restrict__ strong) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    const int r0 = rp[i], r1 = rp[i + 1];
+    int dpos = 0x7fffffff;
+    for (int e = r0 + lane; e < r1; e += kWave)
+      if (ci[e] == (int)i) { dpos = e; break; }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const int o = __shfl_xor(dpos, off, kWave);
+      dpos = o < dpos ? o : dpos;
+    }
+    const double d = (dpos < r1) ? fabs(v[dpos]) : 0.0;
+    const double den = d + 1e-30;
+    double mx = 0.0;
+    for (int e = r0 + lane; e < r1; e += kWave) {
+      const double sv = v[e] / den;
+      if (ci[e] != (int)i && sv < 0.0) mx = fmax(mx, -sv);
+    }
+    mx = wave_max(mx);
+    const double thr = theta * mx;
+    for (int e = r0 + lane; e < r1; e += kWave) {
+      const double sv = v[e] / den;
+      strong[e] = (ci[e] != (int)i && sv < 0.0 && -sv >= thr) ? 1 : 0;
+    }
+  }
+}
+
+__global__ void k_transpose_pos_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                     int* __restrict__ tpos) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    for (int e = rp[i] + lane; e < rp[i + 1]; e += kWave) {
       const int j = ci[e];
       int lo = rp[j], hi = rp[j + 1] - 1, pos = -1;
       while (lo <= hi) {
@@ -207,6 +274,69 @@ __global__ void k_mis_cover(int n, const int* __restrict__ rp, const int* __rest
       else u = 1;
     }
     undecided[i] = u;
+  }
+}
+
+// Wavefront-per-row forms of k_cf_deg, k_mis_select and k_mis_cover: counts
+// summed over the lanes, existence tests by ballot; the same results.
+__global__ void k_cf_deg_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                              int* __restrict__ deg) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    int d = 0;
+    for (int e = rp[i] + lane; e < rp[i + 1]; e += kWave)
+      if (ci[e] != (int)i && in_graph(strong, tpos, e)) ++d;
+    d = wave_sum(d);
+    if (lane == 0) deg[i] = d;
+  }
+}
+
+__global__ void k_mis_select_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                  const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                                  const int* __restrict__ deg, const signed char* __restrict__ cf,
+                                  signed char* __restrict__ newc) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    bool beaten = false;
+    if (cf[i] == kUndecided) {
+      const int di = deg[i];
+      for (int e = rp[i] + lane; e < rp[i + 1] && !beaten; e += kWave) {
+        const int j = ci[e];
+        if (j != (int)i && cf[j] == kUndecided && in_graph(strong, tpos, e) && ranks_above(deg[j], j, di, (int)i))
+          beaten = true;
+      }
+      beaten = __ballot(beaten) != 0;
+      if (lane == 0) newc[i] = beaten ? 0 : 1;
+    } else if (lane == 0) {
+      newc[i] = 0;
+    }
+  }
+}
+
+__global__ void k_mis_cover_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const unsigned char* __restrict__ strong, const int* __restrict__ tpos,
+                                 const signed char* __restrict__ newc, signed char* __restrict__ cf,
+                                 int* __restrict__ undecided) {
+  const int lane = threadIdx.x & (kWave - 1);
+  WAVE_ROW_LOOP(i, n) {
+    int u = 0;
+    const signed char ci0 = cf[i];
+    if (newc[i]) {
+      if (lane == 0) cf[i] = kC;
+    } else if (ci0 == kUndecided) {
+      bool cov = false;
+      for (int e = rp[i] + lane; e < rp[i + 1] && !cov; e += kWave) {
+        const int j = ci[e];
+        if (j != (int)i && newc[j] && in_graph(strong, tpos, e)) cov = true;
+      }
+      cov = __ballot(cov) != 0;
+      if (lane == 0) {
+        if (cov) cf[i] = kF;
+        else u = 1;
+      }
+    }
+    if (lane == 0) undecided[i] = u;
   }
 }
 
@@ -341,6 +471,129 @@ __global__ void k_interp_expand(int n, const int* __restrict__ rp, const int* __
     }
     keys[base + local] = ~0ull;  // spare slot, sorts last
     vals[base + local] = 0.0;
+  }
+}
+
+// Lane q's value of x (q uniform over the wavefront).
+__device__ __forceinline__ double read_lane(double x, int q) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, q);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), q);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ int wave_inclusive_scan(int x, int lane) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int o = __shfl_up(x, off, kWave);
+    if (lane >= off) x += o;
+  }
+  return x;
+}
+
+// Wavefront-per-row form of k_interp_expand for long rows (coarse levels),
+// with the same candidates, weights and slots. The lanes load the row's entries
+// side by side; the diagonal and weak-connection sums fold lane by lane in row
+// order, so they equal the one-thread sums bit for bit. Direct C slots come
+// from ballot ranks; each valid strong F neighbour gets a block of cnt_c slots
+// from a wavefront prefix sum, which its lane fills from that neighbour's row.
+__global__ void k_interp_expand_wave(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                     const double* __restrict__ v, const unsigned char* __restrict__ strong,
+                                     const signed char* __restrict__ cf, const double* __restrict__ sum_strong,
+                                     const int* __restrict__ cnt_c, const int* __restrict__ f2c,
+                                     const long long* __restrict__ cand_ptr, unsigned long long* __restrict__ keys,
+                                     double* __restrict__ vals) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  WAVE_ROW_LOOP(i, n) {
+    const long long base = cand_ptr[i];
+    if (cf[i] == kC) {
+      if (lane == 0) {
+        keys[base] = ((unsigned long long)(unsigned)f2c[i]) << 32;
+        vals[base] = 1.0;
+      }
+      continue;
+    }
+    const int r0 = rp[i], r1 = rp[i + 1];
+    double diag = 0.0, weak_c = 0.0, f_weak = 0.0;
+    for (int e0 = r0; e0 < r1; e0 += kWave) {
+      const int e = e0 + lane;
+      int cat = 0;  // 1 diagonal, 2 weak C, 3 weak F or strong F without C support
+      double x = 0.0;
+      if (e < r1) {
+        const int j = ci[e];
+        if (j == (int)i) {
+          cat = 1;
+          x = v[e];
+        } else if (!strong[e]) {
+          cat = (cf[j] == kC) ? 2 : 3;
+          x = -v[e];
+        } else if (cf[j] == kF && !ff_valid(sum_strong, cnt_c, j)) {
+          cat = 3;
+          x = -v[e];
+        }
+      }
+      const int m = min(kWave, r1 - e0);
+      for (int q = 0; q < m; ++q) {
+        const int c = __builtin_amdgcn_readlane(cat, q);
+        if (c == 0) continue;
+        const double y = read_lane(x, q);
+        if (c == 1) diag += y;
+        else if (c == 2) weak_c += y;
+        else f_weak += y;
+      }
+    }
+    if (!(fabs(diag) > 1e-14)) diag = 1.0;
+    const double ss = sum_strong[i];
+    const double corr = (ss > 1e-14) ? (weak_c + f_weak) / ss : 0.0;
+    unsigned local = 0;
+    for (int e0 = r0; e0 < r1; e0 += kWave) {
+      const int e = e0 + lane;
+      int j = 0;
+      bool take = false;
+      if (e < r1) {
+        j = ci[e];
+        take = j != (int)i && strong[e] && cf[j] == kC;
+      }
+      const unsigned long long m = __ballot(take);
+      if (take) {
+        const unsigned slot = local + (unsigned)__popcll(m & below);
+        double w = finite_or_zero(-v[e] / diag) * (1.0 + corr);
+        w = fmax(w, 0.0);
+        keys[base + slot] = (((unsigned long long)(unsigned)f2c[j]) << 32) | slot;
+        vals[base + slot] = w;
+      }
+      local += (unsigned)__popcll(m);
+    }
+    for (int e0 = r0; e0 < r1; e0 += kWave) {
+      const int e = e0 + lane;
+      int j = 0, cnt = 0;
+      if (e < r1) {
+        j = ci[e];
+        if (j != (int)i && strong[e] && cf[j] == kF && ff_valid(sum_strong, cnt_c, j)) cnt = cnt_c[j];
+      }
+      const int incl = wave_inclusive_scan(cnt, lane);
+      if (cnt > 0) {
+        unsigned slot = local + (unsigned)(incl - cnt);
+        const double factor = finite_or_zero(-v[e] / diag);
+        const double ssj = sum_strong[j];
+        for (int e2 = rp[j]; e2 < rp[j + 1]; ++e2) {
+          const int k = ci[e2];
+          if (k != j && strong[e2] && cf[k] == kC) {
+            const double jw = -v[e2] / ssj;
+            const double c = fmax(factor * jw, 0.0);
+            keys[base + slot] = (((unsigned long long)(unsigned)f2c[k]) << 32) | slot;
+            vals[base + slot] = c;
+            ++slot;
+          }
+        }
+      }
+      local += (unsigned)__shfl(incl, kWave - 1, kWave);
+    }
+    if (lane == 0) {
+      keys[base + local] = ~0ull;  // spare slot, sorts last
+      vals[base + local] = 0.0;
+    }
   }
 }
 
@@ -513,9 +766,28 @@ __global__ void k_spgemm_count(int n, const int* __restrict__ lrp, const int* __
 // One wavefront per output row: lanes cover the right-operand row of each left
 // entry; every product records its key (column, local index) and its operand
 // pair (left entry, right entry) in expansion order.
-#define WAVE_ROW_LOOP(i, n)                                                                  \
-  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave; i < (n); \
-       i += (long long)gridDim.x * blockDim.x / kWave)
+
+// The same expansion with a thread per output row (rows with few products).
+__global__ void k_spgemm_expand_rows(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
+                                     const int* __restrict__ mrp, const int* __restrict__ mci,
+                                     const long long* __restrict__ prod_ptr, unsigned long long* __restrict__ keys,
+                                     int* __restrict__ ex_e, int* __restrict__ ex_f) {
+  ROW_LOOP(i, n) {
+    long long t = prod_ptr[i];
+    unsigned local = 0;
+    for (int e = lrp[i]; e < lrp[i + 1]; ++e) {
+      const int k = lci[e];
+      const int f0 = mrp[k], len = mrp[k + 1] - f0;
+      for (int q = 0; q < len; ++q) {
+        keys[t + q] = (((unsigned long long)(unsigned)mci[f0 + q]) << 32) | (unsigned long long)(local + q);
+        ex_e[t + q] = e;
+        ex_f[t + q] = f0 + q;
+      }
+      t += len;
+      local += (unsigned)len;
+    }
+  }
+}
 
 __global__ void k_spgemm_expand(int n, const int* __restrict__ lrp, const int* __restrict__ lci,
                                 const int* __restrict__ mrp, const int* __restrict__ mci,
@@ -650,28 +922,39 @@ void launch_sym_scale(int n, const int* rp, const int* ci, const double* v, cons
                       hipStream_t s) {
   launch_rows(k_sym_scale, n, s, n, rp, ci, v, dh, vo);
 }
-void launch_strength(int n, const int* rp, const int* ci, const double* v, double theta, unsigned char* strong,
-                     hipStream_t s) {
-  launch_rows(k_strength, n, s, n, rp, ci, v, theta, strong);
+void launch_strength(int n, long long nnz, const int* rp, const int* ci, const double* v, double theta,
+                     unsigned char* strong, hipStream_t s) {
+  if (wave_rows(n, nnz)) launch_waves(k_strength_wave, n, s, n, rp, ci, v, theta, strong);
+  else launch_rows(k_strength, n, s, n, rp, ci, v, theta, strong);
 }
-void launch_transpose_pos(int n, const int* rp, const int* ci, int* tpos, hipStream_t s) {
-  launch_rows(k_transpose_pos, n, s, n, rp, ci, tpos);
+void launch_transpose_pos(int n, long long nnz, const int* rp, const int* ci, int* tpos, hipStream_t s) {
+  if (wave_rows(n, nnz)) launch_waves(k_transpose_pos_wave, n, s, n, rp, ci, tpos);
+  else launch_rows(k_transpose_pos, n, s, n, rp, ci, tpos);
 }
-void launch_cf_split(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos, int* deg,
-                     signed char* is_max, signed char* cf, hipStream_t s) {
-  launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
+void launch_cf_split(int n, long long nnz, const int* rp, const int* ci, const unsigned char* strong,
+                     const int* tpos, int* deg, signed char* is_max, signed char* cf, hipStream_t s) {
+  if (wave_rows(n, nnz)) launch_waves(k_cf_deg_wave, n, s, n, rp, ci, strong, tpos, deg);
+  else launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
   launch_rows(k_cf_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, is_max);
   launch_rows(k_cf_lift, n, s, n, rp, ci, strong, tpos, (const signed char*)is_max, cf);
 }
-void launch_cf_split_iterated(int n, const int* rp, const int* ci, const unsigned char* strong, const int* tpos,
-                              int* deg, signed char* newc, int* undecided, signed char* cf, hipStream_t s) {
-  launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
+void launch_cf_split_iterated(int n, long long nnz, const int* rp, const int* ci, const unsigned char* strong,
+                              const int* tpos, int* deg, signed char* newc, int* undecided, signed char* cf,
+                              hipStream_t s) {
+  const bool wave = wave_rows_mis(n, nnz);
+  if (wave) launch_waves(k_cf_deg_wave, n, s, n, rp, ci, strong, tpos, deg);
+  else launch_rows(k_cf_deg, n, s, n, rp, ci, strong, tpos, deg);
   launch_rows(k_mis_init, n, s, n, cf);
   // the undecided point of highest rank is selected in every round, so at most n rounds
   for (long long round = 0;; ++round) {
     IMPES_REQUIRE(round <= n, "amg setup: independent-set split does not terminate");
-    launch_rows(k_mis_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, (const signed char*)cf, newc);
-    launch_rows(k_mis_cover, n, s, n, rp, ci, strong, tpos, (const signed char*)newc, cf, undecided);
+    if (wave) {
+      launch_waves(k_mis_select_wave, n, s, n, rp, ci, strong, tpos, (const int*)deg, (const signed char*)cf, newc);
+      launch_waves(k_mis_cover_wave, n, s, n, rp, ci, strong, tpos, (const signed char*)newc, cf, undecided);
+    } else {
+      launch_rows(k_mis_select, n, s, n, rp, ci, strong, tpos, (const int*)deg, (const signed char*)cf, newc);
+      launch_rows(k_mis_cover, n, s, n, rp, ci, strong, tpos, (const signed char*)newc, cf, undecided);
+    }
     if (sum_i32(undecided, n, s) == 0) break;
   }
 }
@@ -691,10 +974,14 @@ void launch_interp_count(int n, const int* rp, const int* ci, const unsigned cha
                          const double* sum_strong, const int* cnt_c, int* cand, hipStream_t s) {
   launch_rows(k_interp_count, n, s, n, rp, ci, strong, cf, sum_strong, cnt_c, cand);
 }
-void launch_interp_expand(int n, const int* rp, const int* ci, const double* v, const unsigned char* strong,
-                          const signed char* cf, const double* sum_strong, const int* cnt_c, const int* f2c,
-                          const long long* cand_ptr, unsigned long long* keys, double* vals, hipStream_t s) {
-  launch_rows(k_interp_expand, n, s, n, rp, ci, v, strong, cf, sum_strong, cnt_c, f2c, cand_ptr, keys, vals);
+void launch_interp_expand(int n, long long nnz, const int* rp, const int* ci, const double* v,
+                          const unsigned char* strong, const signed char* cf, const double* sum_strong,
+                          const int* cnt_c, const int* f2c, const long long* cand_ptr, unsigned long long* keys,
+                          double* vals, hipStream_t s) {
+  if (wave_rows(n, nnz))
+    launch_waves(k_interp_expand_wave, n, s, n, rp, ci, v, strong, cf, sum_strong, cnt_c, f2c, cand_ptr, keys, vals);
+  else
+    launch_rows(k_interp_expand, n, s, n, rp, ci, v, strong, cf, sum_strong, cnt_c, f2c, cand_ptr, keys, vals);
 }
 void launch_interp_reduce(int n, const int* rp, const int* ci, const signed char* cf, const int* f2c,
                           const long long* cand_ptr, const unsigned long long* keys, const double* vals,
@@ -717,9 +1004,12 @@ void launch_transpose_finish(int nnz, const unsigned long long* keys, const doub
 void launch_spgemm_count(int n, const int* lrp, const int* lci, const int* mrp, int* cnt, hipStream_t s) {
   launch_rows(k_spgemm_count, n, s, n, lrp, lci, mrp, cnt);
 }
-void launch_spgemm_expand(int n, const int* lrp, const int* lci, const int* mrp, const int* mci,
+void launch_spgemm_expand(int n, long long left_nnz, long long products, const int* lrp, const int* lci, const int* mrp, const int* mci,
                           const long long* prod_ptr, unsigned long long* keys, int* ex_e, int* ex_f, hipStream_t s) {
-  launch_waves(k_spgemm_expand, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys, ex_e, ex_f);
+  if (wave_rows_expand(n, left_nnz, products))
+    launch_waves(k_spgemm_expand, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys, ex_e, ex_f);
+  else
+    launch_rows(k_spgemm_expand_rows, n, s, n, lrp, lci, mrp, mci, prod_ptr, keys, ex_e, ex_f);
 }
 void launch_spgemm_unique_count(int n, const long long* prod_ptr, const unsigned long long* keys, int* cnt,
                                 hipStream_t s) {

@@ -591,6 +591,14 @@ int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_hos
   });
 }
 
+int impes_model_amg_setup_seconds(const impes_model* m, double* out) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && out, "null pointer");
+    const Hierarchy* h = m->m.hierarchy();
+    *out = h ? h->last_setup_seconds : 0.0;
+  });
+}
+
 int impes_ktimer_config(const char* kernel, int enable) {
   return guarded(__func__, [&] { ktimer_config(kernel, enable != 0); });
 }

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -147,32 +148,100 @@ __device__ __forceinline__ T block_max(T v, T* lds, T init) {
   return r;
 }
 
+// ---------------------------------------------------------------- setup memory
+// A hierarchy build makes several hundred device allocations, and every hipFree
+// also waits for the device. While a PoolScope is open on the calling thread,
+// DevBuf allocations of at most kPoolSmall bytes are carved from 32 MiB slabs
+// and larger ones are separate blocks. A small block released under the scope
+// only lowers its slab's live count (slab space is never handed out twice); a
+// large one becomes a spare that a later allocation of the scope may take (at
+// most twice the size asked for). A spare's previous users were enqueued
+// earlier, so the scope's owner must put all device work of the scope on one
+// stream, and close the scope after synchronising it: closing frees the spares
+// and every slab without live blocks. Outside a scope, blocks of the pool are
+// released at once (hipFree, or the slab when it empties).
+class SetupPool {
+ public:
+  static constexpr size_t kPoolSmall = size_t(1) << 20;
+  static constexpr size_t kSlab = size_t(32) << 20;
+  SetupPool();
+  SetupPool(const SetupPool&) = delete;
+  SetupPool& operator=(const SetupPool&) = delete;
+  ~SetupPool();
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void open();
+  void close();
+  size_t slab_bytes() const;  // device memory held in slabs
+
+ private:
+  struct Impl;
+  Impl* impl();
+  std::unique_ptr<Impl> impl_;
+};
+
+// The pool that DevBuf allocations use on this thread (null outside a scope).
+std::shared_ptr<SetupPool> current_setup_pool();
+
+// Waits for s before buffers that its queued work uses are released. Under a
+// PoolScope releases are deferred to the scope's end, so there it waits for nothing.
+inline void sync_before_release(hipStream_t s) {
+  if (!current_setup_pool()) IMPES_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+class PoolScope {
+ public:
+  explicit PoolScope(std::shared_ptr<SetupPool> pool);
+  ~PoolScope();
+  PoolScope(const PoolScope&) = delete;
+  PoolScope& operator=(const PoolScope&) = delete;
+
+ private:
+  std::shared_ptr<SetupPool> pool_, prev_;
+};
+
 // ---------------------------------------------------------------- device buffers
-// Minimal RAII owner of a device allocation (hipMalloc); used by handles that own
-// hierarchy / workspace memory for their lifetime.
+// Minimal RAII owner of a device allocation (hipMalloc, or a setup pool's block
+// when allocated under a PoolScope); used by handles that own hierarchy /
+// workspace memory for their lifetime.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  std::shared_ptr<SetupPool> pool;  // set when the block came from a setup pool
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), pool(std::move(o.pool)) { o.p = nullptr; o.n = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; pool = std::move(o.pool);
+      o.p = nullptr; o.n = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool) pool->release(p);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    pool.reset();
   }
   void alloc(size_t count) {
     if (count == n && p) return;
     release();
     if (count == 0) return;
-    IMPES_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    std::shared_ptr<SetupPool> sp = current_setup_pool();
+    if (sp) {
+      p = static_cast<T*>(sp->alloc(count * sizeof(T)));
+      pool = std::move(sp);
+    } else {
+      IMPES_HIP_CHECK(hipMalloc((void**)&p, count * sizeof(T)));
+    }
     n = count;
   }
   // grow-only allocation (keeps the buffer when already large enough)

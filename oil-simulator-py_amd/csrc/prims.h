@@ -12,13 +12,10 @@ namespace impes {
 void scan_i32_to_i64(const int* in, long long* out, long long n, hipStream_t s);
 void scan_i32_to_i32(const int* in, int* out, long long n, hipStream_t s);
 
-// Sort every segment [seg[i], seg[i+1]) of `keys` ascending. max_len is the
-// largest segment length (host value, e.g. from max_segment_length()).
-void segmented_sort_u64(unsigned long long* keys, const long long* seg, long long nseg, long long max_len,
-                        hipStream_t s);
+// Sort every segment [seg[i], seg[i+1]) of `keys` ascending. Synchronises the
+// stream once (bin sizes).
+void segmented_sort_u64(unsigned long long* keys, const long long* seg, long long nseg, hipStream_t s);
 
-// max over i of seg[i+1]-seg[i]  (synchronises the stream, returns host value)
-long long max_segment_length(const long long* seg, long long nseg, hipStream_t s);
 // sum of an int32 array (synchronises, returns host value)
 long long sum_i32(const int* x, long long n, hipStream_t s);
 // count of int8 entries equal to v (synchronises)

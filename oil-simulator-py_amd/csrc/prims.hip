@@ -111,83 +111,145 @@ void scan_impl(const int* in, TO* out, long long n, hipStream_t s) {
   IMPES_LAUNCH_CHECK();
   k_tile_scan<TO><<<(unsigned)nb, kScanBlock, 0, s>>>(in, n, bsum.p, out);
   IMPES_LAUNCH_CHECK();
-  // bsum must outlive the kernels: synchronise before it is freed.
-  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  sync_before_release(s);  // bsum must outlive the kernels
 }
 
 // ---------------------------------------------------------------- segmented sort
-// One wavefront per segment, keys staged in LDS and bitonic-sorted over the
-// next power of two >= segment length (padding with all-ones keys).
-template <int CAP>
-__global__ __launch_bounds__(kWave) void k_segsort_lds(unsigned long long* __restrict__ keys,
-                                                        const long long* __restrict__ seg, long long nseg) {
-  __shared__ unsigned long long sk[CAP];
-  for (long long sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+// Segments are binned by length (k_seg_bin), and each bin is sorted by a
+// kernel sized for it: a thread per segment of <= 16 keys (registers), else a
+// workgroup per segment with the keys staged in LDS and bitonic-sorted over the
+// next power of two >= the segment length (padding with all-ones keys), from
+// one wavefront for <= 64 keys up to 1024 threads for <= 8192 keys (64 KiB of
+// LDS); longer segments sort the same way through a global scratch region per
+// workgroup. Bins only choose the kernel: each
+// segment is sorted on its own, so the result does not depend on the binning.
+constexpr int kSortBins = 7;  // <= 16, 64, 256, 1024, 4096, 8192 keys, and longer
+
+__device__ __forceinline__ int sort_bin(long long len) {
+  return len <= 16 ? 0 : len <= 64 ? 1 : len <= 256 ? 2 : len <= 1024 ? 3 : len <= 4096 ? 4 : len <= 8192 ? 5 : 6;
+}
+
+// Segments of at most 16 keys: a thread per segment, bitonic network over 16
+// registers (padding with all-ones keys).
+__global__ void k_segsort_regs(unsigned long long* __restrict__ keys, const long long* __restrict__ seg,
+                               const int* __restrict__ list, int count) {
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < count;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long sg = list[q];
     const long long b = seg[sg];
     const int len = (int)(seg[sg + 1] - b);
-    if (len <= 1 || len > CAP) continue;
-    int p2 = 2;
-    while (p2 < len) p2 <<= 1;
-    for (int t = threadIdx.x; t < p2; t += kWave) sk[t] = (t < len) ? keys[b + t] : ~0ull;
-    __syncthreads();
-    for (int size = 2; size <= p2; size <<= 1) {
+    unsigned long long r[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) r[t] = (t < len) ? keys[b + t] : ~0ull;
+#pragma unroll
+    for (int size = 2; size <= 16; size <<= 1) {
+#pragma unroll
       for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int t = threadIdx.x; t < p2; t += kWave) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
           const int partner = t ^ stride;
           if (partner > t) {
             const bool asc = (t & size) == 0;
-            unsigned long long a = sk[t], c = sk[partner];
-            if ((a > c) == asc) { sk[t] = c; sk[partner] = a; }
+            const unsigned long long a = r[t], c = r[partner];
+            if ((a > c) == asc) { r[t] = c; r[partner] = a; }
           }
         }
-        __syncthreads();
       }
     }
-    for (int t = threadIdx.x; t < len; t += kWave) keys[b + t] = sk[t];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (t < len) keys[b + t] = r[t];
+  }
+}
+
+__global__ void k_fill_zero(int* __restrict__ counts, int n, long long* __restrict__ m) {
+  if (threadIdx.x < n) counts[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *m = 0;
+}
+
+// Bin lists: each wavefront claims its slots with one atomic per bin present
+// in it (a single counter per bin would otherwise take one atomic per segment).
+__global__ void k_seg_bin(const long long* __restrict__ seg, long long nseg, int* __restrict__ counts,
+                          long long* __restrict__ long_max, int* __restrict__ lists) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned long long below = lane ? (~0ull >> (kWave - lane)) : 0ull;
+  for (long long i0 = blockIdx.x * (long long)blockDim.x + (threadIdx.x & ~(kWave - 1)); i0 < nseg;
+       i0 += (long long)gridDim.x * blockDim.x) {
+    const long long i = i0 + lane;
+    int bin = -1;
+    long long len = 0;
+    if (i < nseg) {
+      len = seg[i + 1] - seg[i];
+      if (len > 1) bin = sort_bin(len);
+    }
+    for (int b = 0; b < kSortBins; ++b) {
+      const unsigned long long m = __ballot(bin == b);
+      if (!m) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&counts[b], __popcll(m));
+      base = __shfl(base, leader, kWave);
+      if (bin == b) lists[(long long)b * nseg + base + __popcll(m & below)] = (int)i;
+    }
+    if (bin == kSortBins - 1) atomicMax((unsigned long long*)long_max, (unsigned long long)len);
+  }
+}
+
+template <class K>
+__device__ __forceinline__ void bitonic_block(K* sk, long long p2) {
+  for (long long size = 2; size <= p2; size <<= 1) {
+    for (long long stride = size >> 1; stride > 0; stride >>= 1) {
+      for (long long t = threadIdx.x; t < p2; t += blockDim.x) {
+        const long long partner = t ^ stride;
+        if (partner > t) {
+          const bool asc = (t & size) == 0;
+          const K a = sk[t], c = sk[partner];
+          if ((a > c) == asc) { sk[t] = c; sk[partner] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void k_segsort_bin(unsigned long long* __restrict__ keys,
+                                                    const long long* __restrict__ seg, const int* __restrict__ list,
+                                                    int count) {
+  __shared__ unsigned long long sk[CAP];
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const long long sg = list[q];
+    const long long b = seg[sg];
+    const int len = (int)(seg[sg + 1] - b);
+    int p2 = 2;
+    while (p2 < len) p2 <<= 1;
+    for (int t = threadIdx.x; t < p2; t += NT) sk[t] = (t < len) ? keys[b + t] : ~0ull;
+    __syncthreads();
+    bitonic_block(sk, (long long)p2);
+    for (int t = threadIdx.x; t < len; t += NT) keys[b + t] = sk[t];
     __syncthreads();
   }
 }
 
-// Segments longer than the LDS capacity: one block walks them one at a time and
-// bitonic-sorts through a global scratch buffer of the padded length.
+// Segments longer than the LDS bins: workgroup g sorts list entries g, g + grid,
+// ... in scratch[g * p2max, (g + 1) * p2max).
 __global__ __launch_bounds__(1024) void k_segsort_global(unsigned long long* __restrict__ keys,
-                                                         const long long* __restrict__ seg, long long nseg,
-                                                         int min_len, unsigned long long* __restrict__ scratch) {
-  for (long long sg = 0; sg < nseg; ++sg) {
+                                                         const long long* __restrict__ seg,
+                                                         const int* __restrict__ list, int count, long long p2max,
+                                                         unsigned long long* __restrict__ scratch) {
+  unsigned long long* sk = scratch + blockIdx.x * p2max;
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const long long sg = list[q];
     const long long b = seg[sg];
     const long long len = seg[sg + 1] - b;
-    if (len < min_len) continue;
     long long p2 = 2;
     while (p2 < len) p2 <<= 1;
-    for (long long t = threadIdx.x; t < p2; t += blockDim.x) scratch[t] = (t < len) ? keys[b + t] : ~0ull;
+    for (long long t = threadIdx.x; t < p2; t += blockDim.x) sk[t] = (t < len) ? keys[b + t] : ~0ull;
     __syncthreads();
-    for (long long size = 2; size <= p2; size <<= 1) {
-      for (long long stride = size >> 1; stride > 0; stride >>= 1) {
-        for (long long t = threadIdx.x; t < p2; t += blockDim.x) {
-          const long long partner = t ^ stride;
-          if (partner > t) {
-            const bool asc = (t & size) == 0;
-            unsigned long long a = scratch[t], c = scratch[partner];
-            if ((a > c) == asc) { scratch[t] = c; scratch[partner] = a; }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (long long t = threadIdx.x; t < len; t += blockDim.x) keys[b + t] = scratch[t];
+    bitonic_block(sk, p2);
+    for (long long t = threadIdx.x; t < len; t += blockDim.x) keys[b + t] = sk[t];
     __syncthreads();
   }
-}
-
-__global__ void k_seg_maxlen(const long long* __restrict__ seg, long long nseg, long long* __restrict__ part) {
-  __shared__ long long lds[256 / kWave];
-  long long m = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nseg; i += (long long)gridDim.x * blockDim.x) {
-    long long l = seg[i + 1] - seg[i];
-    m = l > m ? l : m;
-  }
-  m = block_max<long long>(m, lds, 0);
-  if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
 __global__ void k_sum_i32(const int* __restrict__ x, long long n, long long* __restrict__ part) {
@@ -224,38 +286,44 @@ long long fold_host(const DevBuf<long long>& part, bool is_max, hipStream_t s) {
 void scan_i32_to_i64(const int* in, long long* out, long long n, hipStream_t s) { scan_impl<long long>(in, out, n, s); }
 void scan_i32_to_i32(const int* in, int* out, long long n, hipStream_t s) { scan_impl<int>(in, out, n, s); }
 
-void segmented_sort_u64(unsigned long long* keys, const long long* seg, long long nseg, long long max_len,
-                        hipStream_t s) {
-  if (nseg <= 0 || max_len <= 1) return;
-  const int grid = (int)(nseg < 65536 ? nseg : 65536);
-  if (max_len <= 64) {
-    k_segsort_lds<64><<<grid, kWave, 0, s>>>(keys, seg, nseg);
-  } else if (max_len <= 256) {
-    k_segsort_lds<256><<<grid, kWave, 0, s>>>(keys, seg, nseg);
-  } else if (max_len <= 1024) {
-    k_segsort_lds<1024><<<grid, kWave, 0, s>>>(keys, seg, nseg);
-  } else {
-    k_segsort_lds<4096><<<grid, kWave, 0, s>>>(keys, seg, nseg);
-  }
+void segmented_sort_u64(unsigned long long* keys, const long long* seg, long long nseg, hipStream_t s) {
+  if (nseg <= 0) return;
+  IMPES_REQUIRE(nseg < (1LL << 31), "segmented sort: too many segments");
+  DevBuf<int> counts, lists;
+  DevBuf<long long> long_max;
+  counts.alloc(kSortBins);
+  long_max.alloc(1);
+  lists.alloc((size_t)kSortBins * nseg);
+  k_fill_zero<<<1, 64, 0, s>>>(counts.p, kSortBins, long_max.p);
   IMPES_LAUNCH_CHECK();
-  if (max_len > 4096) {
+  k_seg_bin<<<grid_for(nseg, 256, 2048), 256, 0, s>>>(seg, nseg, counts.p, long_max.p, lists.p);
+  IMPES_LAUNCH_CHECK();
+  struct { int c[kSortBins]; long long m; } h;
+  IMPES_HIP_CHECK(hipMemcpyAsync(h.c, counts.p, sizeof(h.c), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipMemcpyAsync(&h.m, long_max.p, sizeof(h.m), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  auto grid = [](int c) { return c < 65536 ? c : 65536; };
+  const int* L = lists.p;
+  if (h.c[0]) k_segsort_regs<<<grid_for(h.c[0], 256, 8192), 256, 0, s>>>(keys, seg, L, h.c[0]);
+  if (h.c[1]) k_segsort_bin<64, 64><<<grid(h.c[1]), 64, 0, s>>>(keys, seg, L + nseg, h.c[1]);
+  if (h.c[2]) k_segsort_bin<256, 128><<<grid(h.c[2]), 128, 0, s>>>(keys, seg, L + 2 * nseg, h.c[2]);
+  if (h.c[3]) k_segsort_bin<1024, 256><<<grid(h.c[3]), 256, 0, s>>>(keys, seg, L + 3 * nseg, h.c[3]);
+  if (h.c[4]) k_segsort_bin<4096, 512><<<grid(h.c[4]), 512, 0, s>>>(keys, seg, L + 4 * nseg, h.c[4]);
+  if (h.c[5]) k_segsort_bin<8192, 1024><<<grid(h.c[5]), 1024, 0, s>>>(keys, seg, L + 5 * nseg, h.c[5]);
+  IMPES_LAUNCH_CHECK();
+  if (h.c[6]) {
     long long p2 = 2;
-    while (p2 < max_len) p2 <<= 1;
+    while (p2 < h.m) p2 <<= 1;
+    // at most 2^27 scratch keys (1 GiB) at a time
+    long long g = (1LL << 27) / p2;
+    g = g < 1 ? 1 : (g > h.c[6] ? h.c[6] : g);
+    g = g > 1024 ? 1024 : g;
     DevBuf<unsigned long long> scratch;
-    scratch.alloc((size_t)p2);
-    k_segsort_global<<<1, 1024, 0, s>>>(keys, seg, nseg, 4097, scratch.p);
+    scratch.alloc((size_t)(g * p2));
+    k_segsort_global<<<(unsigned)g, 1024, 0, s>>>(keys, seg, L + 6 * nseg, h.c[6], p2, scratch.p);
     IMPES_LAUNCH_CHECK();
-    IMPES_HIP_CHECK(hipStreamSynchronize(s));
   }
-}
-
-long long max_segment_length(const long long* seg, long long nseg, hipStream_t s) {
-  if (nseg <= 0) return 0;
-  DevBuf<long long> part;
-  part.alloc(kPartBlocks);
-  k_seg_maxlen<<<kPartBlocks, 256, 0, s>>>(seg, nseg, part.p);
-  IMPES_LAUNCH_CHECK();
-  return fold_host(part, true, s);
+  sync_before_release(s);  // the bin lists and the scratch must outlive the kernels
 }
 
 long long sum_i32(const int* x, long long n, hipStream_t s) {

@@ -148,6 +148,7 @@ SIGNATURES = {
     "impes_model_eval_props": [_P, _L, _P, _P, _P, _P, _P],
     "impes_model_eval_pvt": [_P, _I, _L, _P, _P, _P],
     "impes_model_amg_levels": [_P, _PI, _PI, _PI, _I],
+    "impes_model_amg_setup_seconds": [_P, _PD],
     "impes_ktimer_config": [C.c_char_p, _I],
     "impes_ktimer_read": [_PD, _PD, C.POINTER(_L)],
 }

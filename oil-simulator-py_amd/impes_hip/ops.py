@@ -407,3 +407,9 @@ class ImpesModel:
         nnz = (C.c_int * max_levels)()
         call("impes_model_amg_levels", self._h, C.byref(nl), sizes, nnz, max_levels)
         return [(sizes[i], nnz[i]) for i in range(min(nl.value, max_levels))]
+
+    def amg_setup_seconds(self) -> float:
+        """Wall time of the last hierarchy build (0.0 before the first solve)."""
+        v = C.c_double()
+        call("impes_model_amg_setup_seconds", self._h, C.byref(v))
+        return v.value

@@ -186,6 +186,31 @@ def test_hierarchy_update_values(dev):
     _compare_hierarchy(hg, ho)
 
 
+def test_repeated_setup_on_one_handle(dev):
+    """A handle set up again (new pattern) equals a fresh handle, and rebuilds do
+    not accumulate device memory: each build's blocks come from the handle's
+    setup pool, and the previous build's blocks are freed when the next build ends."""
+    import impes_hip as H
+
+    mats = [impes_like(20, 20, 10, seed=s)[0] for s in (6, 7)] + [impes_like(24, 16, 8, seed=8)[0]]
+    b = {A.shape[0]: np.random.default_rng(9).standard_normal(A.shape[0]) for A in mats}
+    hg = H.AmgHierarchy()
+    free = []
+    for rep in range(3):
+        for A in mats:
+            hg.setup_csr(*to_dev(A, dev))
+            fresh = H.AmgHierarchy()
+            fresh.setup_csr(*to_dev(A, dev))
+            x = hg.vcycle(t(b[A.shape[0]], dev), precision=64)
+            y = fresh.vcycle(t(b[A.shape[0]], dev), precision=64)
+            assert torch.equal(x, y)
+            del fresh
+        torch.cuda.synchronize()
+        free.append(torch.cuda.mem_get_info()[0])
+    # the third round holds no more than the second (32 MiB slab granularity)
+    assert free[2] >= free[1] - (64 << 20)
+
+
 @pytest.mark.parametrize("dims", [(16, 12, 6), (12, 10, 8)])
 def test_stencil_hierarchy_equals_csr_hierarchy(dev, dims):
     """nz = 6 runs the FP32 level-0 sweeps one cell per thread, nz = 8 four cells
