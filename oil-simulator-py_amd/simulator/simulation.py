@@ -76,7 +76,7 @@ class Simulator:
         p.coarsest_size = int(self._amg_cfg.get("coarsest_size", 200))
         for key, typ in (("equil_mode", int), ("coarse_pin", int), ("trunc", float), ("pmax", int),
                          ("coarsening", int), ("dominant", float), ("pre_smooth", int), ("post_smooth", int),
-                         ("omega", float)):
+                         ("omega", float), ("dense_max", int)):
             if key in self._amg_cfg:
                 setattr(p, key, typ(self._amg_cfg[key]))
         self._model.set_amg(p)
