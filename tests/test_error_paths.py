@@ -6,7 +6,7 @@ above one (fluid.py:84-85); the mirrors keep those. The device path has no CPU
 fallback: the simulator and the AMG refuse a CPU device. At the C ABI every
 entry point returns a negative status with a message instead of failing inside
 a kernel; the calls below are rejected by argument checks that run before any
-HIP call."""
+HIP call. The last test covers the host logic of the pressure solve's start."""
 import ctypes as C
 
 import pytest
@@ -91,3 +91,22 @@ def test_c_abi_rejects_bad_arguments_before_any_device_call():
         assert lib.impes_amg_vcycle(None, 64, dummy, dummy, 1, None) < 0
     finally:
         assert lib.impes_amg_destroy(h) == 0
+
+
+def test_pressure_initial_guess_modes():
+    """The FGMRES start of the simulator's pressure solve (host logic only)."""
+    import torch
+
+    from simulator.simulation import Simulator
+
+    sim = Simulator.__new__(Simulator)
+    sim._p_hist = None
+    p = torch.tensor([10.0, 20.0], dtype=torch.float64)
+    assert sim._initial_guess(p, 2.0, {}) is p  # no accepted step yet
+    sim._p_hist = (torch.tensor([8.0, 21.0], dtype=torch.float64), 1.0)
+    # p + (p - p_last) * dt / dt_last with dt / dt_last = 2
+    assert torch.equal(sim._initial_guess(p, 2.0, {}), torch.tensor([14.0, 18.0], dtype=torch.float64))
+    assert sim._initial_guess(p, 2.0, {"initial_guess": "previous"}) is p
+    assert torch.equal(sim._initial_guess(p, 2.0, {"initial_guess": "zero"}), torch.zeros(2, dtype=torch.float64))
+    sim._p_hist = (torch.zeros(3, dtype=torch.float64), 1.0)  # another grid: no extrapolation
+    assert sim._initial_guess(p, 2.0, {}) is p
