@@ -44,13 +44,16 @@ class Fgmres {
   double* host_ = nullptr;      // pinned scalars: [0] blocking reads, [2..3] per-iteration estimates
   double* host_dev_ = nullptr;  // device address of host_ (the estimate is stored there by the device)
   hipEvent_t est_ev_[2] = {nullptr, nullptr};
-  // Lookahead (IMPES_FGMRES_LOOKAHEAD=1): iteration j is enqueued before the host
-  // waits for the residual estimate of iteration j-1, so the GPU never idles on the
-  // convergence test; a stop after j-1 ignores iteration j's work (it only writes
-  // Hessenberg column j, g[j..j+1] and basis vectors j+1, none of which the solution
-  // for j columns reads). Off by default: on the 1M-cell case the wasted speculative
-  // iteration costs more than the wait it hides (56.0M vs 53.9M cell-steps/s).
+  // Convergence tests are deferred: iteration j + 1 is enqueued before the host
+  // waits for the residual estimate of iteration j whenever the estimates so far
+  // predict that j cannot end the cycle, so the GPU does not idle on those tests.
+  // A stop after j then ignores iteration j + 1's work (it only writes Hessenberg
+  // column j + 1, g[j+1..j+2] and basis vector j + 2, none of which the solution
+  // for j + 1 columns reads). IMPES_FGMRES_LOOKAHEAD=1 defers every test (on the
+  // 1M-cell case the ignored last iteration then costs more than the waits it
+  // hides: 56.0M vs 53.9M cell-steps/s); IMPES_FGMRES_LOOKAHEAD=0 defers none.
   bool lookahead_ = false;
+  bool no_defer_ = false;
 
  public:
   Fgmres();

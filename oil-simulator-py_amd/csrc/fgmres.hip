@@ -209,9 +209,12 @@ __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
 
 }  // namespace
 
+constexpr double kDeferMargin = 4.0;
+
 Fgmres::Fgmres() {
   const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
   lookahead_ = e && e[0] == '1';
+  no_defer_ = e && e[0] == '0';
 }
 
 Fgmres::~Fgmres() {
@@ -329,11 +332,14 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
     // the estimate of iteration jj lands in pinned slot host_[2 + (jj & 1)]; its
     // event is waited on later
     auto post = [&](int jj) { IMPES_HIP_CHECK(hipEventRecord(est_ev_[jj & 1], s)); };
-    auto done = [&](int jj) {  // iteration jj (0-based) ends the cycle?
+    auto read_est = [&](int jj) {
       IMPES_HIP_CHECK(hipEventSynchronize(est_ev_[jj & 1]));
-      const double est = ((volatile double*)host_)[2 + (jj & 1)];
-      return !(est > tol) || its0 + jj + 1 >= maxit;
+      return ((volatile double*)host_)[2 + (jj & 1)];
     };
+    auto stops = [&](int jj, double est) { return !(est > tol) || its0 + jj + 1 >= maxit; };
+    // last two residual estimates read (the cycle's initial residual first)
+    double e1 = beta, e2 = -1.0;
+    int pending = -1;  // iteration whose estimate is still to be read
     for (int j = 0; j < m; ++j) {
       double* vj = V + (long long)j * n;
       double* zj = Z + (long long)j * n;
@@ -361,19 +367,31 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
       IMPES_LAUNCH_CHECK();
       normalize(V + (long long)(j + 1) * n, w_.p, sc + 3);
       post(j);
-      if (!lookahead_) {
-        k = j + 1;
-        if (done(j)) break;
-        continue;
-      }
-      if (j > 0 && done(j - 1)) {  // iteration j was speculative
-        k = j;
-        break;
+      if (pending >= 0) {  // iteration j - 1 was not tested before j was enqueued
+        const double e = read_est(pending);
+        e2 = e1;
+        e1 = e;
+        pending = -1;
+        if (stops(j - 1, e)) {  // iteration j was speculative
+          k = j;
+          break;
+        }
       }
       k = j + 1;
-      if (j == m - 1) {
-        (void)done(j);  // the cycle ends here anyway; wait for the copy slot
+      // Test iteration j only after enqueuing j + 1 while it cannot end the cycle
+      // by prediction: the last estimate times the last contraction stays above
+      // kDeferMargin x tol. The GPU then does not wait for the host between the
+      // two iterations; a wrong prediction costs one ignored iteration.
+      bool defer = !no_defer_ && j < m - 1 && its0 + j + 2 < maxit;
+      if (defer && !lookahead_) defer = e2 > 0.0 && e1 * std::min(1.0, e1 / e2) > kDeferMargin * tol;
+      if (defer) {
+        pending = j;
+        continue;
       }
+      const double e = read_est(j);
+      e2 = e1;
+      e1 = e;
+      if (stops(j, e)) break;
     }
     its = its0 + k;
     k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
