@@ -374,6 +374,37 @@ def test_graph_replay_matches_eager_vcycle(dev, precision):
     assert rel(z, ho.precondition(vv)) < (1e-11 if precision == 64 else 1e-4)
 
 
+@pytest.mark.parametrize("restart", [30, 6])
+def test_deferred_convergence_tests_change_nothing(dev, restart, monkeypatch):
+    """FGMRES enqueues a step before testing the previous one while the estimates
+    predict no stop. Whatever the deferral (predictive by default, none with
+    IMPES_FGMRES_LOOKAHEAD=0, every test with =1), a speculative step after the
+    stop is ignored: the same bits and iteration counts, also across restarts
+    (restart 6)."""
+    import impes_hip as H
+
+    dims = (24, 20, 8)
+    A, d, ox, oy, oz = impes_like(*dims, seed=12)
+    b = A @ (2e7 + 1e5 * np.random.default_rng(6).standard_normal(A.shape[0]))
+    hg = H.AmgHierarchy()
+    hg.setup_st7(dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev))
+    out = {}
+    for mode in (None, "0", "1"):
+        if mode is None:
+            monkeypatch.delenv("IMPES_FGMRES_LOOKAHEAD", raising=False)
+        else:
+            monkeypatch.setenv("IMPES_FGMRES_LOOKAHEAD", mode)
+        solver = H.FgmresSolver()  # reads the switch when created
+        x = torch.zeros(A.shape[0], dtype=torch.float64, device=dev)
+        st = solver.solve_st7(hg, dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev), t(1.0 / d, dev), t(b, dev),
+                              x, precision=32, rtol=1e-10, restart=restart, maxit=300)
+        assert st["converged"], (mode, st)
+        out[mode] = (x.cpu().numpy(), st["iterations"], st["restarts"])
+    for mode in ("0", "1"):
+        assert np.array_equal(out[mode][0], out[None][0]), mode
+        assert out[mode][1:] == out[None][1:], (mode, out[mode][1:], out[None][1:])
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("dims", [(60, 60, 1), (20, 20, 10)])
 def test_fgmres_matches_direct_solve(dev, precision, dims):
