@@ -181,7 +181,6 @@ class Hierarchy {
 
   bool graphs_ = true;
   bool transposed_ = false;  // eager cycles of vcycle64/32 with nu1 and nu2 swapped
-  hipStream_t cap_stream_ = nullptr;       // capture only; replays run on the caller's stream
   hipGraphExec_t vexec_[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64 V-cycle
 
   AmgParams prm_;

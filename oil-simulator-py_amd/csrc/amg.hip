@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <vector>
 
 #include "amg_kernels.h"
@@ -288,10 +289,24 @@ Hierarchy::Hierarchy(const AmgParams& p) : prm_(p) {
   graphs_ = !(g && g[0] == '0');
 }
 
-Hierarchy::~Hierarchy() {
-  drop_graphs();
-  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+Hierarchy::~Hierarchy() { drop_graphs(); }
+
+namespace {
+// Capture-only stream (nothing runs on it): one per thread and device, created on
+// first use and kept until the process ends. Creating a stream takes ~4 ms, which
+// every new hierarchy (each environment reset, for one) would otherwise pay.
+hipStream_t capture_stream() {
+  thread_local std::map<int, hipStream_t> streams;
+  int dev = 0;
+  IMPES_HIP_CHECK(hipGetDevice(&dev));
+  auto it = streams.find(dev);
+  if (it != streams.end()) return it->second;
+  hipStream_t cs = nullptr;
+  IMPES_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  streams[dev] = cs;
+  return cs;
 }
+}  // namespace
 
 void Hierarchy::drop_graphs() {
   for (auto& e : vexec_) {
@@ -301,19 +316,19 @@ void Hierarchy::drop_graphs() {
 }
 
 void Hierarchy::capture_vcycle(int precision) {
-  if (!cap_stream_) IMPES_HIP_CHECK(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+  hipStream_t cs = capture_stream();
   Level& L0 = levels_[0];
   hipGraph_t g = nullptr;
-  IMPES_HIP_CHECK(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal));
+  IMPES_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   try {
-    if (precision == 32) vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, cap_stream_);
-    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, cap_stream_);
+    if (precision == 32) vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, cs);
+    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, cs);
   } catch (...) {
-    (void)hipStreamEndCapture(cap_stream_, &g);
+    (void)hipStreamEndCapture(cs, &g);
     if (g) (void)hipGraphDestroy(g);
     throw;
   }
-  IMPES_HIP_CHECK(hipStreamEndCapture(cap_stream_, &g));
+  IMPES_HIP_CHECK(hipStreamEndCapture(cs, &g));
   hipGraphExec_t e = nullptr;
   const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
