@@ -48,10 +48,13 @@ def parse():
 
 
 def make_sim(dims, precision):
-    from simulator.cases import baseline_case, build
+    """The benchmark case with the shared solver settings (simulator.cases.BENCH_AMG,
+    the ones the GPU parity tests run): FGMRES to a 1e-8 relative residual and an
+    estimated relative pressure error below 1e-8."""
+    from simulator.cases import BENCH_AMG, baseline_case, build
 
     cfg, fields = baseline_case(2, dims=dims)
-    cfg["simulation"]["amg"] = {"tol": 1e-8, "precision": precision, "max_levels": 25, "max_iter": 400}
+    cfg["simulation"]["amg"] = dict(BENCH_AMG, precision=precision)
     return cfg, build(cfg, fields, "cuda")
 
 
@@ -79,12 +82,13 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
             setup = {"first_step_ms": 1e3 * (time.perf_counter() - t0),
                      "hierarchy_setup_ms": 1e3 * sim._model.amg_setup_seconds()}
     torch.cuda.synchronize()
-    its = []
+    its, est = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         if not sim.run_step(dt):
             raise RuntimeError("timed step failed (pressure solve did not converge)")
-        its.append(int(sim.last_solve.get("iterations", 0)))
+        its.append((int(sim.last_solve.get("iterations", 0)), int(sim.last_solve.get("checks", 0))))
+        est.append(float(sim.last_solve.get("err_estimate", 0.0)))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     kts = {}
@@ -96,6 +100,7 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
         torch.cuda.synchronize()
         kts[kernel] = H.ktimer_read()
         H.ktimer_reset(enable=False)
+    setup["max_error_estimate"] = max(est) if est else None
     return sim, el, its, kts, setup
 
 
@@ -312,6 +317,7 @@ def main():
     import numpy as np
     import torch
     import impes_hip as H
+    from simulator.cases import BENCH_AMG
 
     H.load()  # fails loudly when the HIP library is missing
     if a.calibrate_copy:
@@ -338,7 +344,8 @@ def main():
         extra = {"fp64_value": ncell * a.steps / el64, "fp64_ms_per_step": 1e3 * el64 / a.steps,
                  "speedup_fp32_vs_fp64": el64 / el32,
                  "fp32_vs_fp64_pressure_max_rel": float((p32 - p64).abs().max() / p64.abs().max()),
-                 "fgmres_iterations_fp64": its64, "setup_fp64": setup64}
+                 "fgmres_iterations_fp64": [i for i, _ in its64], "error_checks_fp64": [c for _, c in its64],
+                 "setup_fp64": setup64}
         del sim64
 
     # rooflines of the dominant kernel and of the others named, on a fresh run of
@@ -384,7 +391,8 @@ def main():
            "vs_baseline": None, "dtype": "f64 (FP32 AMG preconditioner)", "data": "synthetic",
            "config": {"workload": "three-phase gas-injection black-oil IMPES, 100x100x100 (1M cells)",
                       "dims": list(dims), "cells": ncell, "dt_days": 1.0, "wells": 9, "rel_residual": 1e-8,
-                      "amg_levels": [list(x) for x in levels], "fgmres_iterations": its32,
+                      "solver": dict(BENCH_AMG), "amg_levels": [list(x) for x in levels],
+                      "fgmres_iterations": [i for i, _ in its32], "error_checks": [c for _, c in its32],
                       "parallelism": "single GPU"},
            "roofline": roof, "cpu_baseline": cpu, "setup": setup32}
     out.update(extra)

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define IMPES_ABI_VERSION 1
+#define IMPES_ABI_VERSION 2
 
 typedef struct impes_amg impes_amg;
 typedef struct impes_solver impes_solver;
@@ -150,10 +150,22 @@ typedef struct {
   double rel_residual; /* ||D^-1 (b - A x)|| / ||D^-1 b|| at exit */
   double bnorm;        /* ||D^-1 b|| */
   double r0_rel;       /* relative residual of the initial guess */
+  int checks;          /* pressure-error checks run (impes_solver_set_error_tol) */
+  double err_estimate; /* last check's ||M r||_inf / ||x||_inf */
 } impes_solve_stats;
 
 int impes_solver_create(impes_solver** out);
 int impes_solver_destroy(impes_solver* solver);
+/* Pressure-error check of the FGMRES solves on this handle (0 = off, the default).
+ * No counterpart in the reference, whose amg_solve stops on the residual alone
+ * (classical_amg.py:1136-1142). Once the relative residual reaches rtol, one
+ * preconditioner application to the scaled residual, z = M D^-1 (b - A x),
+ * estimates the error A^-1 (b - A x); x += z refines the answer, and the solve
+ * ends when ||z||_inf <= etol/2 * ||x||_inf. Otherwise the residual target is
+ * lowered by the ratio of estimate to residual and the iteration continues; the
+ * ratio is kept for the next solve on the handle, which starts from the
+ * predicted target. */
+int impes_solver_set_error_tol(impes_solver* solver, double etol);
 
 /* FP64 restarted FGMRES on D^-1 A x = D^-1 b, right-preconditioned by one AMG
  * V-cycle (amg may be NULL: unpreconditioned). dinv = 1/diag(A). Takes over
@@ -243,6 +255,9 @@ int impes_model_assemble(impes_model* m, double dt, const double* p, const doubl
  * Replaces the "amg" branch of simulation.py:904-929. */
 int impes_model_solve_pressure(impes_model* m, int precision, double rtol, int restart, int maxit,
                                const double* p_prev, double* p_new, impes_solve_stats* stats, void* stream);
+/* The pressure-error check of impes_solver_set_error_tol for this model's solves
+ * (0 = off, the default); the north star's "pressure to 1e-8 relative" bar. */
+int impes_model_set_error_tol(impes_model* m, double etol);
 /* Jacobi-preconditioned CG, reference semantics (absolute stop on sqrt(r.z), x0 = 0):
  * Simulator._solve_pressure_cg_pytorch, simulation.py:1662-1697. converged/iterations: host. */
 int impes_model_solve_cg(impes_model* m, double tol, int maxit, double* p_new, int* converged, int* iterations,

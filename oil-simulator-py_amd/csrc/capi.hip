@@ -365,6 +365,16 @@ static void fill_stats(impes_solve_stats* out, const SolveStats& st) {
   out->rel_residual = st.rel_residual;
   out->bnorm = st.bnorm;
   out->r0_rel = st.r0_rel;
+  out->checks = st.checks;
+  out->err_estimate = st.err_estimate;
+}
+
+int impes_solver_set_error_tol(impes_solver* solver, double etol) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(solver, "null solver");
+    IMPES_REQUIRE(etol >= 0.0 && etol < 1.0, "error tolerance must be in [0, 1)");
+    if (etol != solver->f.error_tol()) solver->f.set_error_tol(etol);
+  });
 }
 
 int impes_fgmres_csr(impes_solver* solver, impes_amg* amg, int precision, int n, int nnz, const int* rowptr,
@@ -525,6 +535,14 @@ int impes_model_solve_pressure(impes_model* m, int precision, double rtol, int r
     SolveStats st;
     m->m.solve(precision, rtol, restart, maxit, p_new, st, s);
     fill_stats(stats, st);
+  });
+}
+
+int impes_model_set_error_tol(impes_model* m, double etol) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m, "null pointer");
+    IMPES_REQUIRE(etol >= 0.0 && etol < 1.0, "error tolerance must be in [0, 1)");
+    if (etol != m->m.error_tol()) m->m.set_error_tol(etol);
   });
 }
 

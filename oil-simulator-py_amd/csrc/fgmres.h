@@ -27,10 +27,25 @@ struct SolveStats {
   double rel_residual = 0; // true ||D^-1(b-Ax)|| / ||D^-1 b|| at exit
   double bnorm = 0;        // ||D^-1 b||
   double r0_rel = 0;       // relative residual of the initial guess
+  int checks = 0;          // pressure-error checks run (each one preconditioner application)
+  double err_estimate = 0; // last check's ||M r||_inf / ||x||_inf (estimated relative pressure error)
 };
 
 class Fgmres {
  public:
+  // Pressure-error check (0 = off). Once the relative residual reaches rtol, the
+  // preconditioner applied to the scaled residual, z = M r, estimates the error
+  // A^-1 (b - A x) (M approximates A^-1 D); x += z refines the answer, and the
+  // solve ends when ||z||_inf <= kErrAccept * etol * ||x||_inf. Otherwise the
+  // residual target is lowered by the ratio of estimate to residual and the
+  // iteration continues from the corrected x. The ratios of the last two checks
+  // are kept: the next solve starts from the target their larger one predicts
+  // (the ratio alternates from step to step when the pressure does).
+  void set_error_tol(double etol) {
+    etol_ = etol;
+    err_ratio_[0] = err_ratio_[1] = 0.0;
+  }
+  double error_tol() const { return etol_; }
   void solve(const Operator& A, Hierarchy* M, int precision, const double* b, double* x, bool use_x0, double rtol,
              int restart, int maxit, SolveStats& st, hipStream_t s);
 
@@ -54,6 +69,8 @@ class Fgmres {
   // hides: 56.0M vs 53.9M cell-steps/s); IMPES_FGMRES_LOOKAHEAD=0 defers none.
   bool lookahead_ = false;
   bool no_defer_ = false;
+  double etol_ = 0.0;
+  double err_ratio_[2] = {0.0, 0.0};  // estimate / relative residual at the last two checks
 
  public:
   Fgmres();

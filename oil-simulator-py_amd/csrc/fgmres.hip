@@ -210,6 +210,8 @@ __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
 }  // namespace
 
 constexpr double kDeferMargin = 4.0;
+constexpr double kErrAccept = 0.5;   // accept an error estimate up to this fraction of etol
+constexpr double kErrTarget = 0.25;  // residual targets aim the estimate at this fraction of etol
 
 Fgmres::Fgmres() {
   const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
@@ -323,88 +325,123 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
   double beta = std::sqrt(read(sc + 4));
   st.r0_rel = beta / bnorm;
   int its = 0;
-  while (beta > tol && its < maxit && std::isfinite(beta)) {
-    k_init_cycle<<<1, 1, 0, s>>>(L, sm);
-    IMPES_LAUNCH_CHECK();
-    normalize(V, V, sc + 0);
-    int k = 0;
-    const int its0 = its;
-    // the estimate of iteration jj lands in pinned slot host_[2 + (jj & 1)]; its
-    // event is waited on later
-    auto post = [&](int jj) { IMPES_HIP_CHECK(hipEventRecord(est_ev_[jj & 1], s)); };
-    auto read_est = [&](int jj) {
-      IMPES_HIP_CHECK(hipEventSynchronize(est_ev_[jj & 1]));
-      return ((volatile double*)host_)[2 + (jj & 1)];
-    };
-    auto stops = [&](int jj, double est) { return !(est > tol) || its0 + jj + 1 >= maxit; };
-    // last two residual estimates read (the cycle's initial residual first)
-    double e1 = beta, e2 = -1.0;
-    int pending = -1;  // iteration whose estimate is still to be read
-    for (int j = 0; j < m; ++j) {
-      double* vj = V + (long long)j * n;
-      double* zj = Z + (long long)j * n;
-      double* est = host_dev_ + 2 + (j & 1);
-      precondition_apply(vj, zj);
-      // CGS2: h = V^T w; then w -= V h with h2 = V^T w in one sweep; then w -= V h2
-      // with ||w||^2 and the rotations
-      const int kj = j + 1;
-      if (kj <= kMaxFusedGs) {
-        launch_gs_pass<0>(kj, w_.p, V, n, nullptr, work_.p, s);
-        k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h());
-        IMPES_LAUNCH_CHECK();
-        launch_gs_pass<1>(kj, w_.p, V, n, sm + L.h(), work_.p, s);
-        k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h2());
-        IMPES_LAUNCH_CHECK();
-        launch_gs_pass<2>(kj, w_.p, V, n, sm + L.h2(), work_.p, s);
-      } else {
-        launch_multidot_f64(V, n, kj, w_.p, n, sm + L.h(), work_.p, s);
-        launch_gs_update_f64(w_.p, V, n, kj, sm + L.h(), n, s);
-        launch_multidot_f64(V, n, kj, w_.p, n, sm + L.h2(), work_.p, s);
-        launch_gs_update_f64(w_.p, V, n, kj, sm + L.h2(), n, s);
-        launch_dot_partial_f64(w_.p, w_.p, n, work_.p, kMultiDotBlocks, s);
-      }
-      k_arnoldi<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kMultiDotBlocks, est);
+  const bool check = etol_ > 0.0 && M != nullptr;
+  // residual target of the Krylov cycles: tol, or lower when the error check
+  // predicts that tol leaves too large a pressure error
+  double target = tol;
+  const double ratio = std::max(err_ratio_[0], err_ratio_[1]);
+  if (check && ratio > 0.0) target = std::min(tol, kErrTarget * etol_ / ratio * bnorm);
+  bool ok = false;
+  for (;;) {
+    while (beta > target && its < maxit && std::isfinite(beta)) {
+      k_init_cycle<<<1, 1, 0, s>>>(L, sm);
       IMPES_LAUNCH_CHECK();
-      normalize(V + (long long)(j + 1) * n, w_.p, sc + 3);
-      post(j);
-      if (pending >= 0) {  // iteration j - 1 was not tested before j was enqueued
-        const double e = read_est(pending);
+      normalize(V, V, sc + 0);
+      int k = 0;
+      const int its0 = its;
+      // the estimate of iteration jj lands in pinned slot host_[2 + (jj & 1)]; its
+      // event is waited on later
+      auto post = [&](int jj) { IMPES_HIP_CHECK(hipEventRecord(est_ev_[jj & 1], s)); };
+      auto read_est = [&](int jj) {
+        IMPES_HIP_CHECK(hipEventSynchronize(est_ev_[jj & 1]));
+        return ((volatile double*)host_)[2 + (jj & 1)];
+      };
+      auto stops = [&](int jj, double est) { return !(est > target) || its0 + jj + 1 >= maxit; };
+      // last two residual estimates read (the cycle's initial residual first)
+      double e1 = beta, e2 = -1.0;
+      int pending = -1;  // iteration whose estimate is still to be read
+      for (int j = 0; j < m; ++j) {
+        double* vj = V + (long long)j * n;
+        double* zj = Z + (long long)j * n;
+        double* est = host_dev_ + 2 + (j & 1);
+        precondition_apply(vj, zj);
+        // CGS2: h = V^T w; then w -= V h with h2 = V^T w in one sweep; then w -= V h2
+        // with ||w||^2 and the rotations
+        const int kj = j + 1;
+        if (kj <= kMaxFusedGs) {
+          launch_gs_pass<0>(kj, w_.p, V, n, nullptr, work_.p, s);
+          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h());
+          IMPES_LAUNCH_CHECK();
+          launch_gs_pass<1>(kj, w_.p, V, n, sm + L.h(), work_.p, s);
+          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h2());
+          IMPES_LAUNCH_CHECK();
+          launch_gs_pass<2>(kj, w_.p, V, n, sm + L.h2(), work_.p, s);
+        } else {
+          launch_multidot_f64(V, n, kj, w_.p, n, sm + L.h(), work_.p, s);
+          launch_gs_update_f64(w_.p, V, n, kj, sm + L.h(), n, s);
+          launch_multidot_f64(V, n, kj, w_.p, n, sm + L.h2(), work_.p, s);
+          launch_gs_update_f64(w_.p, V, n, kj, sm + L.h2(), n, s);
+          launch_dot_partial_f64(w_.p, w_.p, n, work_.p, kMultiDotBlocks, s);
+        }
+        k_arnoldi<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kMultiDotBlocks, est);
+        IMPES_LAUNCH_CHECK();
+        normalize(V + (long long)(j + 1) * n, w_.p, sc + 3);
+        post(j);
+        if (pending >= 0) {  // iteration j - 1 was not tested before j was enqueued
+          const double e = read_est(pending);
+          e2 = e1;
+          e1 = e;
+          pending = -1;
+          if (stops(j - 1, e)) {  // iteration j was speculative
+            k = j;
+            break;
+          }
+        }
+        k = j + 1;
+        // Test iteration j only after enqueuing j + 1 while it cannot end the cycle
+        // by prediction: the last estimate times the last contraction stays above
+        // kDeferMargin x tol. The GPU then does not wait for the host between the
+        // two iterations; a wrong prediction costs one ignored iteration.
+        bool defer = !no_defer_ && j < m - 1 && its0 + j + 2 < maxit;
+        if (defer && !lookahead_) defer = e2 > 0.0 && e1 * std::min(1.0, e1 / e2) > kDeferMargin * target;
+        if (defer) {
+          pending = j;
+          continue;
+        }
+        const double e = read_est(j);
         e2 = e1;
         e1 = e;
-        pending = -1;
-        if (stops(j - 1, e)) {  // iteration j was speculative
-          k = j;
-          break;
-        }
+        if (stops(j, e)) break;
       }
-      k = j + 1;
-      // Test iteration j only after enqueuing j + 1 while it cannot end the cycle
-      // by prediction: the last estimate times the last contraction stays above
-      // kDeferMargin x tol. The GPU then does not wait for the host between the
-      // two iterations; a wrong prediction costs one ignored iteration.
-      bool defer = !no_defer_ && j < m - 1 && its0 + j + 2 < maxit;
-      if (defer && !lookahead_) defer = e2 > 0.0 && e1 * std::min(1.0, e1 / e2) > kDeferMargin * tol;
-      if (defer) {
-        pending = j;
-        continue;
-      }
-      const double e = read_est(j);
-      e2 = e1;
-      e1 = e;
-      if (stops(j, e)) break;
+      its = its0 + k;
+      k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
+      IMPES_LAUNCH_CHECK();
+      launch_lincomb_add_f64(x, Z, n, k, sm + L.y(), n, s);
+      residual(A, x, b, V, s);
+      launch_dot_f64(V, V, n, sc + 4, work_.p, s);
+      beta = std::sqrt(read(sc + 4));
+      ++st.restarts;
     }
-    its = its0 + k;
-    k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
-    IMPES_LAUNCH_CHECK();
-    launch_lincomb_add_f64(x, Z, n, k, sm + L.y(), n, s);
+    if (!check || !(beta <= tol)) {
+      ok = beta <= tol;
+      break;
+    }
+    // error check: V holds the scaled residual D^-1 (b - A x); z = M V lands in
+    // Z's first column (w receives D^-1 A z, unused)
+    launch_fill_f64(sc + 6, 1.0, 1, s);
+    normalize(w_.p, V, sc + 6);
+    precondition_apply(w_.p, Z);
+    launch_amax_f64(Z, n, sc + 6, work_.p, s);
+    launch_axpy_f64(x, 1.0, Z, n, s);
+    launch_amax_f64(x, n, sc + 7, work_.p, s);
     residual(A, x, b, V, s);
     launch_dot_f64(V, V, n, sc + 4, work_.p, s);
-    beta = std::sqrt(read(sc + 4));
-    ++st.restarts;
+    IMPES_HIP_CHECK(hipMemcpyAsync(host_ + 4, sc + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    IMPES_HIP_CHECK(hipStreamSynchronize(s));
+    const double zmax = host_[6], xmax = host_[7];
+    const double est = (zmax > 0.0) ? zmax / std::max(xmax, 1e-300) : 0.0;
+    err_ratio_[1] = err_ratio_[0];
+    err_ratio_[0] = est / std::max(beta / bnorm, 1e-300);
+    beta = std::sqrt(host_[4]);
+    ++st.checks;
+    st.err_estimate = est;
+    ok = est <= kErrAccept * etol_ && beta <= tol;
+    if (ok || its >= maxit || !std::isfinite(beta)) break;
+    target = std::min(target, beta * kErrTarget * etol_ / std::max(est, 1e-300));
   }
   st.iterations = its;
   st.rel_residual = beta / bnorm;
-  st.converged = (beta <= tol) ? 1 : 0;
+  st.converged = ok ? 1 : 0;
 }
 
 }  // namespace impes

@@ -54,6 +54,9 @@ class Model {
   void set_wells(const std::vector<WellDesc>& wells);
   void set_amg_params(const AmgParams& p) { amg_prm_ = p; amg_.reset(); }
   void reset_hierarchy() { amg_.reset(); }
+  // pressure-error check of the FGMRES solve (relative, infinity norm; 0 = off)
+  void set_error_tol(double etol) { fgmres_.set_error_tol(etol); }
+  double error_tol() const { return fgmres_.error_tol(); }
 
   // Pressure equation at the current state: face coefficients, diagonal, rhs, 1/diag.
   void assemble(double dt, const double* p, const double* sw, const double* sg, hipStream_t s);

@@ -53,6 +53,8 @@ class SolveStats(C.Structure):
         ("rel_residual", C.c_double),
         ("bnorm", C.c_double),
         ("r0_rel", C.c_double),
+        ("checks", C.c_int),
+        ("err_estimate", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -128,6 +130,7 @@ SIGNATURES = {
     "impes_amg_set_graphs": [_P, _I],
     "impes_solver_create": [C.POINTER(_P)],
     "impes_solver_destroy": [_P],
+    "impes_solver_set_error_tol": [_P, _D],
     "impes_fgmres_csr": [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I, C.POINTER(SolveStats), _P],
     "impes_fgmres_st7": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I,
                          C.POINTER(SolveStats), _P],
@@ -142,6 +145,7 @@ SIGNATURES = {
     "impes_model_report_len": [_P, _PI],
     "impes_model_assemble": [_P, _D, _P, _P, _P, _P],
     "impes_model_solve_pressure": [_P, _I, _D, _I, _I, _P, _P, C.POINTER(SolveStats), _P],
+    "impes_model_set_error_tol": [_P, _D],
     "impes_model_solve_cg": [_P, _D, _I, _P, _PI, _PI, _P],
     "impes_model_saturation": [_P, _D, _P, _P, _P, _P, _P, _P, _P, _PD, _P],
     "impes_model_export": [_P, _I, _P, _P],

@@ -266,8 +266,9 @@ class FgmresSolver:
         self.close()
 
     def solve_csr(self, amg, rowptr, col, val, dinv, b, x, *, precision=64, use_x0=False, rtol=1e-8,
-                  restart=30, maxit=300) -> dict:
+                  restart=30, maxit=300, etol=0.0) -> dict:
         st = SolveStats()
+        call("impes_solver_set_error_tol", self._h, float(etol))
         n = rowptr.numel() - 1
         call("impes_fgmres_csr", self._h, amg.handle if amg is not None else None, int(precision), n, val.numel(),
              ptr(_i32(rowptr)), ptr(_i32(col)), ptr(val), ptr(dinv), ptr(b), ptr(x), 1 if use_x0 else 0,
@@ -275,8 +276,9 @@ class FgmresSolver:
         return st.as_dict()
 
     def solve_st7(self, amg, dims, d, ox, oy, oz, dinv, b, x, *, precision=64, use_x0=False, rtol=1e-8,
-                  restart=30, maxit=300) -> dict:
+                  restart=30, maxit=300, etol=0.0) -> dict:
         st = SolveStats()
+        call("impes_solver_set_error_tol", self._h, float(etol))
         nx, ny, nz = (int(v) for v in dims)
         call("impes_fgmres_st7", self._h, amg.handle if amg is not None else None, int(precision), nx, ny, nz,
              ptr(d), ptr(ox), ptr(oy), ptr(oz), ptr(dinv), ptr(b), ptr(x), 1 if use_x0 else 0, float(rtol),
@@ -362,8 +364,12 @@ class ImpesModel:
     def assemble(self, dt, p, sw, sg):
         call("impes_model_assemble", self._h, float(dt), ptr(p), ptr(sw), ptr(sg), stream_ptr())
 
-    def solve_pressure(self, p_prev, p_new, *, precision=32, rtol=1e-8, restart=30, maxit=300) -> dict:
+    def solve_pressure(self, p_prev, p_new, *, precision=32, rtol=1e-8, restart=30, maxit=300,
+                       etol=0.0) -> dict:
+        """FGMRES to the relative residual rtol; etol > 0 adds the pressure-error
+        check (estimated relative error in the infinity norm held to etol)."""
         st = SolveStats()
+        call("impes_model_set_error_tol", self._h, float(etol))
         call("impes_model_solve_pressure", self._h, int(precision), float(rtol), int(restart), int(maxit),
              ptr(p_prev), ptr(p_new), C.byref(st), stream_ptr())
         return st.as_dict()

@@ -175,8 +175,12 @@ class Simulator:
             tol = float(cfg.get("tol", self.sim_params.get("cg_tolerance", 1e-6)))
             prec = str(cfg.get("precision", "fp32")).lower()
             precision = 64 if prec in ("fp64", "64", "float64", "double") else 32
+            # "error_tol": estimated relative pressure error (infinity norm) the solve
+            # must also reach; defaults to tol, 0 stops on the residual alone
+            etol = float(cfg.get("error_tol", tol))
             st = self._model.solve_pressure(self._initial_guess(p, dt, cfg), p_new, precision=precision, rtol=tol,
-                                            restart=int(cfg.get("restart", 30)), maxit=int(cfg.get("max_iter", 300)))
+                                            restart=int(cfg.get("restart", 30)), maxit=int(cfg.get("max_iter", 300)),
+                                            etol=etol)
             converged = bool(st["converged"])
         else:
             tol = float(self.sim_params.get("cg_tolerance", 1e-6))

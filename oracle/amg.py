@@ -470,71 +470,117 @@ class Hierarchy:
         return self.dh * x
 
 
-def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, restart=30, maxit=300):
+ERR_ACCEPT = 0.5  # the error check accepts an estimate up to this fraction of etol
+ERR_TARGET = 0.25  # residual targets aim the estimate at this fraction of etol
+
+
+def fgmres(A: sp.csr_matrix, b: np.ndarray, x0=None, precond=None, rtol=1e-8, restart=30, maxit=300, etol=0.0,
+           memo=None):
     """Right-preconditioned restarted FGMRES on D^-1 A x = D^-1 b (same algorithm
     as the HIP solver): classical Gram-Schmidt with a second, always-run pass
-    (CGS2). Returns (x, stats)."""
+    (CGS2). Returns (x, stats).
+
+    etol > 0 adds a pressure-error check once the relative residual is at rtol:
+    z = M r (one preconditioner application to the scaled residual) estimates the
+    error A^-1 r, since M approximates A^-1 D; the estimate is applied (x += z, a
+    refinement step whose error is (I - M A) e) and accepted when
+    ||z||_inf <= ERR_ACCEPT * etol * ||x||_inf. Otherwise the residual target is
+    lowered by the ratio of estimate to residual and the iteration continues
+    (restarting from the corrected x). ``memo`` (a dict kept across solves)
+    holds the ratios of the last two checks; the next solve starts from the
+    target their larger one predicts."""
     A = sp.csr_matrix(A)
     dinv = 1.0 / A.diagonal()
     bs = dinv * b
     bnorm = np.linalg.norm(bs)
     x = np.zeros_like(b) if x0 is None else x0.astype(np.float64).copy()
     if bnorm == 0:
-        return np.zeros_like(b), {"iterations": 0, "converged": True, "rel_residual": 0.0}
+        return np.zeros_like(b), {"iterations": 0, "converged": True, "rel_residual": 0.0, "checks": 0,
+                                  "err_estimate": 0.0}
+    check = etol > 0 and precond is not None
+    memo = {} if memo is None else memo
     tol = rtol * bnorm
+    target = tol
+    ratio = max(memo.get("ratios", [0.0]))
+    if check and ratio > 0:
+        target = min(tol, ERR_TARGET * etol / ratio * bnorm)
     r = dinv * (b - A @ x)
     beta = np.linalg.norm(r)
     its = 0
+    checks, est, ok = 0, 0.0, True
     m = restart
-    while beta > tol and its < maxit:
-        V = np.zeros((m + 1, b.size))
-        Z = np.zeros((m, b.size))
-        H = np.zeros((m + 1, m))
-        cs = np.zeros(m)
-        sn = np.zeros(m)
-        g = np.zeros(m + 1)
-        g[0] = beta
-        V[0] = r / beta
-        k = 0
-        for j in range(m):
-            Z[j] = precond(V[j]) if precond is not None else V[j]
-            w = dinv * (A @ Z[j])
-            h = V[:j + 1] @ w
-            w = w - V[:j + 1].T @ h
-            h2 = V[:j + 1] @ w
-            w = w - V[:j + 1].T @ h2
-            H[:j + 1, j] = h + h2
-            hn = np.linalg.norm(w)
-            H[j + 1, j] = hn
-            for i in range(j):
-                a, bb = H[i, j], H[i + 1, j]
-                H[i, j] = cs[i] * a + sn[i] * bb
-                H[i + 1, j] = -sn[i] * a + cs[i] * bb
-            a, bb = H[j, j], H[j + 1, j]
-            if bb == 0:
-                c, s = 1.0, 0.0
-            elif abs(bb) > abs(a):
-                t = a / bb
-                s = 1 / np.sqrt(1 + t * t)
-                c = s * t
-            else:
-                t = bb / a
-                c = 1 / np.sqrt(1 + t * t)
-                s = c * t
-            cs[j], sn[j] = c, s
-            H[j, j] = c * a + s * bb
-            H[j + 1, j] = 0.0
-            g[j + 1] = -s * g[j]
-            g[j] = c * g[j]
-            V[j + 1] = w / (hn if hn > 0 else 1.0)
-            its += 1
-            k = j + 1
-            if not abs(g[j + 1]) > tol or its >= maxit:
-                break
-        y = np.zeros(k)
-        for i in range(k - 1, -1, -1):
-            y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i] if H[i, i] != 0 else 0.0
-        x = x + Z[:k].T @ y
+    while True:
+        while beta > target and its < maxit:
+            x, k = _fgmres_cycle(A, dinv, x, r, beta, target, precond, m, maxit - its)
+            its += k
+            r = dinv * (b - A @ x)
+            beta = np.linalg.norm(r)
+        if not check or not beta <= tol:
+            ok = bool(beta <= tol)
+            break
+        z = precond(r)
+        x = x + z
+        est = float(np.max(np.abs(z)) / max(np.max(np.abs(x)), 1e-300))
+        checks += 1
+        memo["ratios"] = [est / max(beta / bnorm, 1e-300)] + memo.get("ratios", [])[:1]
         r = dinv * (b - A @ x)
         beta = np.linalg.norm(r)
-    return x, {"iterations": its, "converged": bool(beta <= tol), "rel_residual": beta / bnorm}
+        ok = est <= ERR_ACCEPT * etol
+        if ok or its >= maxit:
+            break
+        target = min(target, beta * ERR_TARGET * etol / est)
+    return x, {"iterations": its, "converged": bool(ok and beta <= tol), "rel_residual": beta / bnorm,
+               "checks": checks, "err_estimate": est}
+
+
+def _fgmres_cycle(A, dinv, x, r, beta, tol, precond, m, maxit):
+    """One restart cycle of at most min(m, maxit) Arnoldi steps from residual r
+    (||r|| = beta). Returns the updated x and the number of steps."""
+    n = x.size
+    V = np.zeros((m + 1, n))
+    Z = np.zeros((m, n))
+    H = np.zeros((m + 1, m))
+    cs = np.zeros(m)
+    sn = np.zeros(m)
+    g = np.zeros(m + 1)
+    g[0] = beta
+    V[0] = r / beta
+    k = 0
+    for j in range(m):
+        Z[j] = precond(V[j]) if precond is not None else V[j]
+        w = dinv * (A @ Z[j])
+        h = V[:j + 1] @ w
+        w = w - V[:j + 1].T @ h
+        h2 = V[:j + 1] @ w
+        w = w - V[:j + 1].T @ h2
+        H[:j + 1, j] = h + h2
+        hn = np.linalg.norm(w)
+        H[j + 1, j] = hn
+        for i in range(j):
+            a, bb = H[i, j], H[i + 1, j]
+            H[i, j] = cs[i] * a + sn[i] * bb
+            H[i + 1, j] = -sn[i] * a + cs[i] * bb
+        a, bb = H[j, j], H[j + 1, j]
+        if bb == 0:
+            c, s = 1.0, 0.0
+        elif abs(bb) > abs(a):
+            t = a / bb
+            s = 1 / np.sqrt(1 + t * t)
+            c = s * t
+        else:
+            t = bb / a
+            c = 1 / np.sqrt(1 + t * t)
+            s = c * t
+        cs[j], sn[j] = c, s
+        H[j, j] = c * a + s * bb
+        H[j + 1, j] = 0.0
+        g[j + 1] = -s * g[j]
+        g[j] = c * g[j]
+        V[j + 1] = w / (hn if hn > 0 else 1.0)
+        k = j + 1
+        if not abs(g[j + 1]) > tol or k >= maxit:
+            break
+    y = np.zeros(k)
+    for i in range(k - 1, -1, -1):
+        y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i] if H[i, i] != 0 else 0.0
+    return x + Z[:k].T @ y, k

@@ -40,7 +40,7 @@ def test_abi_version_and_error_channel():
     import impes_hip
 
     lib = impes_hip.load()
-    assert lib.impes_abi_version() == 1
+    assert lib.impes_abi_version() == 2
     # a call with invalid arguments fails cleanly (argument checks run before any HIP call)
     rc = lib.impes_amg_create(None, None)
     assert rc < 0

@@ -96,37 +96,85 @@ def test_pressure_system_matches_oracle(dev, name):
         sim.run_step(dt)
 
 
-def run_parity(cfg, fields, dt, steps, precision=64, rtol=1e-10, p_tol=1e-8, s_tol=1e-6, check_n=True):
-    sim = gpu_sim(cfg, fields, pressure_solver="amg",
-                  amg={"tol": rtol, "precision": "fp64" if precision == 64 else "fp32", "max_iter": 400})
+def solver_cfg(precision=32, **over):
+    """The benchmark's pressure-solver settings (FP32 or FP64 hierarchy)."""
+    from simulator.cases import BENCH_AMG
+
+    return dict(BENCH_AMG, precision="fp64" if precision == 64 else "fp32", **over)
+
+
+def close(a, b, rel=1e-6, scale=0.0):
+    return math.isclose(a, b, rel_tol=rel, abs_tol=1e-9 * (1.0 + abs(b)) + rel * scale)
+
+
+def run_parity(cfg, fields, dt, steps, amg, p_tol=1e-8, s_tol=1e-6, whole_step=True):
+    """Steps the device simulator with the solver settings `amg`; before each step
+    the oracle takes the simulator's state.
+
+    Pressure: against the oracle's exact solve (p_tol, relative). Transport:
+    against the oracle's transport step evaluated at the simulator's own new
+    pressure: saturations (s_tol, absolute), substep count, component and mass
+    ledger increments (simulation.py:1396-1529) and every well's diagnostics
+    (last_mode, last_q_total, last_surface_rates; well.py:53-55). Whole step
+    (whole_step=True): saturations also against the oracle step with the exact
+    pressure, which holds where the reference's transport is continuous in the
+    pressure at solver-error level. At the end: the cumulative volume balance."""
+    sim = gpu_sim(cfg, fields, pressure_solver="amg", amg=amg)
     orc = oracle_for(cfg, fields)
-    worst = {"p": 0.0, "s": 0.0}
+    worst = {"p": 0.0, "s": 0.0, "s_whole": 0.0}
     prev_cb = {ph: dict(v) for ph, v in sim.component_balance.items()}
+    prev_mb = {ph: dict(v) for ph, v in sim.mass_balance.items()}
+    wells = sim.well_manager.get_wells()
     for s in range(steps):
-        orc.state = np_state(sim)
+        before = np_state(sim)
+        orc.state = before
         rec = orc.step(dt)
         assert sim.run_step(dt)
         st = np_state(sim)
         po = rec["p"]
         dp = np.max(np.abs(st["p"] - po)) / np.max(np.abs(po))
-        ds = max(np.max(np.abs(st["sw"] - rec["sat"]["sw"])), np.max(np.abs(st["sg"] - rec["sat"]["sg"])))
-        worst["p"], worst["s"] = max(worst["p"], dp), max(worst["s"], ds)
+        worst["p"] = max(worst["p"], dp)
         assert dp < p_tol, (s, dp, sim.last_solve)
-        if check_n:
-            # N = ceil(max rat); when max rat sits on an integer (a choked well filling its
-            # window exactly) the two roundings may land either side of it
-            r = rec["sat"]["rat_max"]
-            on_int = abs(r - round(r)) <= 1e-9 * max(1.0, abs(r))
-            assert sim.last_substeps == rec["N"] or (on_int and abs(sim.last_substeps - rec["N"]) <= 1), (
-                s, sim.last_substeps, rec["N"], r)
+        sat = oi.saturation_step(orc.case, before, st["p"], dt, rec["ps"])
+        ds = max(np.max(np.abs(st["sw"] - sat["sw"])), np.max(np.abs(st["sg"] - sat["sg"])))
+        worst["s"] = max(worst["s"], ds)
         assert ds < s_tol, (s, ds)
+        ds_whole = max(np.max(np.abs(st["sw"] - rec["sat"]["sw"])), np.max(np.abs(st["sg"] - rec["sat"]["sg"])))
+        worst["s_whole"] = max(worst["s_whole"], ds_whole)
+        for o, check in ((sat, True), (rec["sat"], whole_step)):
+            # N = ceil(max rat); when max rat sits on an integer (a choked well filling
+            # its window exactly) the two roundings may land either side of it
+            r = o["rat_max"]
+            on_int = abs(r - round(r)) <= 1e-9 * max(1.0, abs(r))
+            assert not check or sim.last_substeps == o["N"] or (on_int and abs(sim.last_substeps - o["N"]) <= 1), (
+                s, sim.last_substeps, o["N"], r)
+        if whole_step:
+            assert ds_whole < s_tol, (s, ds_whole)
         cb = sim.component_balance
         for ph in ("oil", "gas"):
             for io in ("in", "out"):
                 inc = cb[ph][io] - prev_cb[ph][io]
-                ref = rec["sat"]["comp"][f"{ph}_{io}"]
-                assert math.isclose(inc, ref, rel_tol=1e-6, abs_tol=1e-9 * (1 + abs(ref))), (s, ph, io, inc, ref)
+                ref = sat["comp"][f"{ph}_{io}"]
+                assert close(inc, ref), (s, ph, io, inc, ref)
         prev_cb = {ph: dict(v) for ph, v in cb.items()}
+        mb = sim.mass_balance
+        mass = sat["mass"]
+        for ph in ("water", "oil", "gas"):
+            # the ledgers sum clipped per-cell increments; a cell whose increment sits
+            # at rounding level may land on either side of zero: scale by the total
+            scale = 1e-3 * float(mass[ph + "_in"] + mass[ph + "_out"])
+            for io in ("in", "out"):
+                inc = mb[ph][io] - prev_mb[ph][io]
+                ref = float(mass[f"{ph}_{io}"])
+                assert close(inc, ref, scale=scale), (s, ph, io, inc, ref)
+        prev_mb = {ph: dict(v) for ph, v in mb.items()}
+        for w, wd in zip(wells, sat["wells"]):
+            if wd["last_mode"] is not None:
+                assert w.last_mode == wd["last_mode"], (s, w.name, w.last_mode, wd["last_mode"])
+            if wd["last_q_total"] is not None:
+                assert close(w.last_q_total, wd["last_q_total"]), (s, w.name, w.last_q_total, wd["last_q_total"])
+                for ph, v in wd["last_surface_rates"].items():
+                    assert close(w.last_surface_rates[ph], v, scale=abs(wd["last_q_total"])), (s, w.name, ph)
     err = sim.volume_balance_error()
     assert err["water"] < 1e-10 and err["gas"] < 1e-10, err
     return sim, worst
@@ -136,21 +184,42 @@ def run_parity(cfg, fields, dt, steps, precision=64, rtol=1e-10, p_tol=1e-8, s_t
 @pytest.mark.parametrize("name", sorted(META))
 def test_reference_cases_match_oracle(dev, name, precision):
     cfg, dt = golden_config(name)
-    run_parity(cfg, None, dt, steps=3, precision=precision)
+    run_parity(cfg, None, dt, steps=3, amg=solver_cfg(precision))
 
 
-@pytest.mark.parametrize("index,dims,steps", [(0, (60, 60, 1), 8), (1, (20, 20, 6), 4), (2, (16, 16, 10), 1),
-                                              (3, (24, 24, 4), 4)])
+@pytest.mark.parametrize("index,dims,steps", [(0, (60, 60, 1), 8), (1, (20, 20, 6), 6), (2, (16, 16, 10), 4),
+                                              (3, (24, 24, 4), 4), (1, (40, 40, 10), 4), (2, (32, 32, 24), 3)])
 def test_benchmark_cases_match_oracle(dev, index, dims, steps):
-    """The benchmark cases (shrunk) with the production hierarchy (FP32 V-cycles
-    inside FP64 FGMRES). The solve runs to a 1e-10 relative residual: a 1e-8
-    residual in the row-scaled norm leaves pressure errors up to ~1e-7 relative
-    on these near-singular systems, above the 1e-8 pressure bar."""
+    """The benchmark cases (shrunk) with exactly the benchmark's solver settings
+    (FP32 V-cycles inside FP64 FGMRES to a 1e-8 relative residual, extrapolated
+    start, pressure-error check at 1e-8). The residual alone would not do: on the
+    config-1 cases a 1e-8 residual in the row-scaled norm leaves pressure errors
+    up to ~5e-7 relative (cells below the bubble point carry a negative total
+    compressibility), which the error check catches and removes."""
     from simulator.cases import baseline_case
 
     cfg, fields = baseline_case(index, dims=dims)
     dt = cfg["simulation"]["time_step_days"] * 86400.0
-    run_parity(cfg, fields, dt, steps, precision=32, rtol=1e-10)
+    amg = solver_cfg(32)
+    assert cfg["simulation"]["amg"] == (amg if index else solver_cfg(64))
+    # config 2's transport amplifies pressure errors ~1e6-fold next to the gas
+    # injectors (a 1e-12 relative pressure change moves saturations by ~5e-6:
+    # tests/test_oracle_impes.py::test_gas_injection_transport_amplifies_pressure_error),
+    # so its transport is compared at the device pressure only
+    sim, _ = run_parity(cfg, fields, dt, steps, amg=amg, whole_step=index != 2)
+    assert all(r["solve"]["checks"] >= 1 for r in sim.step_log)
+
+
+def test_residual_stop_alone_misses_pressure_bar(dev):
+    """Without the error check (error_tol 0) the 1e-8 residual stop leaves the
+    shrunk config-1 pressure above the 1e-8 bar within a few steps: the check is
+    what holds the bar, not slack in the test."""
+    from simulator.cases import baseline_case
+
+    cfg, fields = baseline_case(1, dims=(20, 20, 6))
+    dt = cfg["simulation"]["time_step_days"] * 86400.0
+    with pytest.raises(AssertionError):
+        run_parity(cfg, fields, dt, 6, amg=solver_cfg(32, error_tol=0.0))
 
 
 def test_reference_cg_solver_path(dev):
