@@ -9,21 +9,9 @@ of the IMPES path and are not provided.
 """
 from __future__ import annotations
 
-import os
-
 import torch
 
 from .pvt import PVTTable
-
-_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-
-
-def _resolve_pvt_path(path: str) -> str:
-    if os.path.exists(path):
-        return path
-    cand = os.path.join(_DATA, os.path.basename(path))
-    return cand if os.path.exists(cand) else path
-
 
 class Fluid:
     def __init__(self, config, reservoir, device=None):
@@ -52,7 +40,7 @@ class Fluid:
             self.pvt = PVTTable(data=config["pvt_table"])
         elif pvt_path:
             try:
-                self.pvt = PVTTable(_resolve_pvt_path(pvt_path))
+                self.pvt = PVTTable(pvt_path)
             except Exception as e:  # the reference keeps running without the table (fluid.py:54-58)
                 print(f"  PVT: could not load '{pvt_path}': {e}")
         rp = config.get("relative_permeability", {})

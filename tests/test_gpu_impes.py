@@ -20,8 +20,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 META = json.load(open(os.path.join(GOLD, "steps.json")))
 FL = np.load(os.path.join(GOLD, "fluid.npz"))
 FLCFG = json.load(open(os.path.join(GOLD, "fluid.json")))
-PVT_SYNTH = os.path.join(os.path.dirname(GOLD), "..", "oil-simulator-py_amd", "simulator", "data",
-                         "pvt_synthetic.json")
+PVT_SYNTH = os.path.join(GOLD, "pvt_synthetic.json")
 
 
 def golden_config(name):

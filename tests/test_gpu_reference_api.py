@@ -4,8 +4,9 @@ Each test restates one test of the reference (file:line cited) with the same
 configuration and the same assertion, constructing Reservoir / WellManager /
 Fluid / Simulator exactly as the reference test does but with device "cuda".
 The reference reads configs/pvt/pvt_synthetic.json and
-configs/gas_injection_3phase.json; the same data come from the package copy of
-the PVT table and from the golden metadata (case "gas3")."""
+configs/gas_injection_3phase.json; the same data come from the fixture copy of
+the PVT table (tests/golden/pvt_synthetic.json) and from the golden metadata
+(case "gas3")."""
 import json
 import os
 
@@ -16,7 +17,7 @@ import torch
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-PVT = os.path.join(HERE, "..", "oil-simulator-py_amd", "simulator", "data", "pvt_synthetic.json")
+PVT = os.path.join(HERE, "golden", "pvt_synthetic.json")
 META = json.load(open(os.path.join(HERE, "golden", "steps.json")))
 
 

@@ -11,8 +11,7 @@ from oracle import impes as oi
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FL = np.load(os.path.join(GOLD, "fluid.npz"))
 FLCFG = json.load(open(os.path.join(GOLD, "fluid.json")))
-PVT_SYNTH = json.load(open(os.path.join(GOLD, "..", "..", "oil-simulator-py_amd", "simulator", "data",
-                                        "pvt_synthetic.json")))
+PVT_SYNTH = json.load(open(os.path.join(GOLD, "pvt_synthetic.json")))
 
 
 def fluid_case(variant):
