@@ -235,14 +235,16 @@ int impes_model_destroy(impes_model* m);
  * simulation.py:52-74 computed on the device. */
 int impes_model_set_rock(impes_model* m, const double* kx, const double* ky, const double* kz, const double* pv,
                          void* stream);
-int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host);
+/* New well set (ReservoirEnv._apply_action changes controls every step, rl_env.py:129-140):
+ * uploaded on `stream` (the host array may be released on return). */
+int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host, void* stream);
 /* Batched environments (the reference steps one reservoir per Simulator inside
  * ReservoirEnv.step, rl_env.py:90-108): the grid holds nx / env_nx independent
  * reservoirs of env_nx x-layers each, stacked along x. Faces between them carry no
  * flow and every reservoir gets its own substep count (the saturation report then
  * ends with those nx / env_nx counts). env_nx 0 or nx: one reservoir. Call before
  * impes_model_set_rock. */
-int impes_model_set_envs(impes_model* m, int env_nx);
+int impes_model_set_envs(impes_model* m, int env_nx, void* stream);
 int impes_model_set_amg(impes_model* m, const impes_amg_params* params);
 int impes_model_report_len(const impes_model* m, int* out);
 /* Pressure equation at (p, sw, sg): Simulator._impes_pressure_step up to the solve

@@ -486,17 +486,17 @@ int impes_model_set_rock(impes_model* m, const double* kx, const double* ky, con
   });
 }
 
-int impes_model_set_envs(impes_model* m, int env_nx) {
+int impes_model_set_envs(impes_model* m, int env_nx, void* stream) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(m, "null model");
-    m->m.set_envs(env_nx);
+    m->m.set_envs(env_nx, as_stream(stream));
   });
 }
 
-int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host) {
+int impes_model_set_wells(impes_model* m, int nwells, const impes_well* wells_host, void* stream) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(m, "null model");
-    m->m.set_wells(wells_from(nwells, wells_host));
+    m->m.set_wells(wells_from(nwells, wells_host), as_stream(stream));
   });
 }
 

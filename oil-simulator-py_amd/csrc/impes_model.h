@@ -49,9 +49,11 @@ class Model {
   // The grid holds nx / env_nx independent reservoirs stacked along x (env_nx x-layers
   // each): no faces between them and a substep count per reservoir (its count in the
   // report tail). 0 or nx = one reservoir. Call before set_rock.
-  void set_envs(int env_nx);
+  void set_envs(int env_nx, hipStream_t s);
   int num_envs() const { return n_env_; }
-  void set_wells(const std::vector<WellDesc>& wells);
+  // new well descriptors, uploaded on stream s (through pinned staging, so the
+  // caller's host array may be released on return)
+  void set_wells(const std::vector<WellDesc>& wells, hipStream_t s);
   void set_amg_params(const AmgParams& p) { amg_prm_ = p; amg_.reset(); }
   void reset_hierarchy() { amg_.reset(); }
   // pressure-error check of the FGMRES solve (relative, infinity norm; 0 = off)
@@ -92,6 +94,17 @@ class Model {
   bool gravity_ = false;
   DevBuf<double> pvt_P_, pvt_col_, pvt_slope_;
   DevBuf<WellDesc> wells_;
+  // pinned staging of the well descriptors; `ev` marks when the last upload has
+  // left it
+  struct Staging {
+    WellDesc* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging();
+  } stage_;
   DevBuf<double> well_sc_;  // per-well terms of the transport step (kWellScratch doubles per well)
   DevBuf<double> mob_w_, mob_o_, mob_g_, rho_w_, rho_o_, rho_g_, pc_;
   DevBuf<double> pcog_, div_w_, div_g_, dsw_, dsg_, qw_, qg_, qo_;

@@ -1,6 +1,8 @@
 // IMPES step kernels and their host orchestration.
 #include "impes_model.h"
 
+#include <cstring>
+
 #include <cmath>
 
 // Products and sums are rounded separately (no fused multiply-add), as in the
@@ -856,22 +858,42 @@ Model::Model(const Grid& g, const FluidParams& f, int pvt_n, const double* pvt_h
   IMPES_HIP_CHECK(hipMemset(qg_.p, 0, n * sizeof(double)));
   IMPES_HIP_CHECK(hipMemset(qo_.p, 0, n * sizeof(double)));
   part_.alloc((size_t)kRedBlocks * kLedgerQ);
-  set_wells(w);
+  set_wells(w, nullptr);
+  IMPES_HIP_CHECK(hipStreamSynchronize(nullptr));
 }
 
-void Model::set_wells(const std::vector<WellDesc>& w) {
+Model::Staging::~Staging() {
+  if (ev) {
+    (void)hipEventSynchronize(ev);
+    (void)hipEventDestroy(ev);
+  }
+  if (p) (void)hipHostFree(p);
+}
+
+void Model::set_wells(const std::vector<WellDesc>& w, hipStream_t s) {
   for (const auto& d : w) IMPES_REQUIRE(d.cell >= 0 && d.cell < N_, "model: well outside the grid");
   nw_ = (int)w.size();
   if (nw_ > 0) {
-    wells_.alloc(nw_);
-    well_sc_.alloc((size_t)kWellScratch * nw_);
-    IMPES_HIP_CHECK(hipMemcpy(wells_.p, w.data(), sizeof(WellDesc) * nw_, hipMemcpyHostToDevice));
+    wells_.reserve(nw_);
+    well_sc_.reserve((size_t)kWellScratch * nw_);
+    if (!stage_.ev) IMPES_HIP_CHECK(hipEventCreateWithFlags(&stage_.ev, hipEventDisableTiming));
+    IMPES_HIP_CHECK(hipEventSynchronize(stage_.ev));  // the last upload has left the staging
+    if (stage_.cap < (size_t)nw_) {
+      if (stage_.p) IMPES_HIP_CHECK(hipHostFree(stage_.p));
+      stage_.p = nullptr;
+      stage_.cap = 0;
+      IMPES_HIP_CHECK(hipHostMalloc((void**)&stage_.p, sizeof(WellDesc) * nw_, hipHostMallocDefault));
+      stage_.cap = nw_;
+    }
+    std::memcpy(stage_.p, w.data(), sizeof(WellDesc) * nw_);
+    IMPES_HIP_CHECK(hipMemcpyAsync(wells_.p, stage_.p, sizeof(WellDesc) * nw_, hipMemcpyHostToDevice, s));
+    IMPES_HIP_CHECK(hipEventRecord(stage_.ev, s));
   }
-  report_.alloc(report_len());
-  IMPES_HIP_CHECK(hipMemset(report_.p, 0, sizeof(double) * report_len()));
+  report_.reserve(report_len());
+  launch_fill_f64(report_.p, 0.0, report_len(), s);
 }
 
-void Model::set_envs(int env_nx) {
+void Model::set_envs(int env_nx, hipStream_t s) {
   IMPES_REQUIRE(env_nx >= 0, "model: env_nx must be >= 0");
   if (env_nx == 0 || env_nx == grid.nx) {
     env_nx_ = 0;
@@ -885,8 +907,8 @@ void Model::set_envs(int env_nx) {
   }
   env_cells_ = (long long)(env_nx_ > 0 ? env_nx_ : grid.nx) * grid.ny * grid.nz;
   amg_.reset();  // the operator's pattern loses the faces between reservoirs
-  report_.alloc(report_len());
-  IMPES_HIP_CHECK(hipMemset(report_.p, 0, sizeof(double) * report_len()));
+  report_.reserve(report_len());
+  launch_fill_f64(report_.p, 0.0, report_len(), s);
 }
 
 void Model::set_rock(const double* kx, const double* ky, const double* kz, const double* pv_in, hipStream_t s) {

@@ -139,8 +139,8 @@ SIGNATURES = {
                            C.POINTER(_P)],
     "impes_model_destroy": [_P],
     "impes_model_set_rock": [_P, _P, _P, _P, _P, _P],
-    "impes_model_set_wells": [_P, _I, C.POINTER(WellStruct)],
-    "impes_model_set_envs": [_P, _I],
+    "impes_model_set_wells": [_P, _I, C.POINTER(WellStruct), _P],
+    "impes_model_set_envs": [_P, _I, _P],
     "impes_model_set_amg": [_P, C.POINTER(AmgParams)],
     "impes_model_report_len": [_P, _PI],
     "impes_model_assemble": [_P, _D, _P, _P, _P, _P],

@@ -344,12 +344,12 @@ class ImpesModel:
     def set_envs(self, env_nx: int):
         """The grid holds nx // env_nx independent reservoirs stacked along x (call
         before set_rock); the saturation report then ends with their substep counts."""
-        call("impes_model_set_envs", self._h, int(env_nx))
+        call("impes_model_set_envs", self._h, int(env_nx), stream_ptr())
         self._report = None
 
     def set_wells(self, wells):
         warr, nw = self._wells(wells)
-        call("impes_model_set_wells", self._h, nw, warr)
+        call("impes_model_set_wells", self._h, nw, warr, stream_ptr())
         self.nwells = nw
         self._report = None
 

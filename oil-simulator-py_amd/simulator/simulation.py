@@ -196,15 +196,18 @@ class Simulator:
     def _initial_guess(self, p, dt, cfg):
         """FGMRES starting pressure. "previous": the current pressure. "extrapolate"
         (default): linear in time through the start pressures of this and the last
-        accepted step, p + (p - p_last) dt / dt_last. Either way the solve runs to
-        the same relative residual; a better start saves iterations."""
+        accepted step, p + (p - p_last) dt / dt_last, when the last step was at
+        least as long as this one; after a shorter step (one the attempt loop cut)
+        the current pressure, since the change over a cut step says little about
+        the next full one. Either way the solve runs to the same stopping test; a
+        better start saves iterations."""
         mode = str(cfg.get("initial_guess", "extrapolate"))
         if mode == "zero":
             return torch.zeros_like(p)
         if mode != "extrapolate" or self._p_hist is None:
             return p
         p_last, dt_last = self._p_hist
-        if p_last.shape != p.shape or not dt_last > 0.0:
+        if p_last.shape != p.shape or not float(dt_last) >= float(dt):
             return p
         return torch.add(p, p - p_last, alpha=float(dt) / float(dt_last))
 

@@ -7,6 +7,9 @@ preconditioned by one RS-AMG V-cycle (FP32 hierarchy unless precision=64),
 stopping on ||D^-1 (b - A x)|| <= tol * ||D^-1 b||; ``method="vcycle"``
 restores the reference's stationary V-cycle iteration (relative residual of
 the equilibrated system). ``amg_solve.last_stats`` holds the solver report.
+``hierarchy="reference"`` builds the reference's hierarchy instead of this
+package's default (see solver.classical_amg). ``error_tol`` > 0 adds the
+FGMRES pressure-error check (estimated relative error, infinity norm).
 """
 from __future__ import annotations
 
@@ -20,17 +23,20 @@ _AMG_CACHE: Dict[Tuple[Any, ...], Any] = {}
 _SOLVER = {}
 
 
-def _entry(A, theta, max_levels, coarsest_size, dev):
+def _entry(A, theta, max_levels, coarsest_size, dev, hierarchy="default"):
     from impes_hip import AmgHierarchy
 
+    if hierarchy not in ("default", "reference"):
+        raise ValueError(f"hierarchy must be 'default' or 'reference', got {hierarchy!r}")
     rp, ci, v, shape = to_device_csr(A, dev)
-    key = (tuple(shape), float(theta), int(max_levels), int(coarsest_size), str(dev))
+    key = (tuple(shape), float(theta), int(max_levels), int(coarsest_size), str(dev), hierarchy)
     ent = _AMG_CACHE.get(key)
     if ent is not None and torch.equal(ent["rp"], rp) and torch.equal(ent["ci"], ci):
         ent["amg"].update_csr(v)
         ent["hit"] = True
     else:
-        h = AmgHierarchy(theta=theta, max_levels=max_levels, coarsest_size=coarsest_size)
+        extra = dict(AmgHierarchy.REFERENCE) if hierarchy == "reference" else {}
+        h = AmgHierarchy(theta=theta, max_levels=max_levels, coarsest_size=coarsest_size, **extra)
         h.setup_csr(rp, ci, v)
         ent = {"amg": h, "rp": rp, "ci": ci, "hit": False}
         _AMG_CACHE[key] = ent
@@ -53,12 +59,14 @@ def amg_solve(
     precision: int = 32,
     restart: int = 30,
     maxiter: Optional[int] = None,
+    hierarchy: str = "default",
+    error_tol: float = 0.0,
 ) -> torch.Tensor:
     """Solve A x = b; returns x on b.device with b.dtype."""
     from impes_hip import FgmresSolver, csr_diagonal
 
     dev = _device(device)
-    ent = _entry(A, theta, max_levels, coarsest_size, dev)
+    ent = _entry(A, theta, max_levels, coarsest_size, dev, hierarchy)
     h = ent["amg"]
     b64 = b.to(dev, dtype=torch.float64).contiguous().reshape(-1)
     if method == "vcycle":
@@ -69,7 +77,8 @@ def amg_solve(
         dinv = 1.0 / csr_diagonal(ent["rp"], ent["ci"], ent["v"])
         x = torch.zeros_like(b64)
         st = solver.solve_csr(h, ent["rp"], ent["ci"], ent["v"], dinv, b64, x, precision=precision, rtol=tol,
-                              restart=restart, maxit=maxiter if maxiter is not None else max(int(max_cycles), 300))
+                              restart=restart, maxit=maxiter if maxiter is not None else max(int(max_cycles), 300),
+                              etol=error_tol)
         st["method"] = "fgmres"
         amg_solve.last_stats = st
     return x.to(b.device, dtype=b.dtype)
@@ -85,6 +94,7 @@ def amg_preconditioner(
     max_levels: int = 10,
     coarsest_size: int = 200,
     device: Optional[str] = "auto",
+    hierarchy: str = "default",
 ) -> Dict[str, Any]:
     """pressure_amg.py:92-160: right preconditioner r -> z ~ A^-1 r (one V-cycle through
     ClassicalAMG.apply), with the reference's theta back-off on a quality test."""
@@ -109,7 +119,8 @@ def amg_preconditioner(
     candidates = [theta] + ([0.2] if theta > 0.2 else []) + [0.15, 0.1]
     best, best_score, chosen = None, float("inf"), None
     for th in candidates:
-        amg = ClassicalAMG(A_csr, theta=th, max_levels=max_levels, coarsest_size=coarsest_size, device=dev)
+        amg = ClassicalAMG(A_csr, theta=th, max_levels=max_levels, coarsest_size=coarsest_size, device=dev,
+                           hierarchy=hierarchy)
         score = assess(amg)
         if score < best_score:
             best, best_score = amg, score

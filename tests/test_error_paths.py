@@ -103,9 +103,12 @@ def test_pressure_initial_guess_modes():
     sim._p_hist = None
     p = torch.tensor([10.0, 20.0], dtype=torch.float64)
     assert sim._initial_guess(p, 2.0, {}) is p  # no accepted step yet
-    sim._p_hist = (torch.tensor([8.0, 21.0], dtype=torch.float64), 1.0)
-    # p + (p - p_last) * dt / dt_last with dt / dt_last = 2
-    assert torch.equal(sim._initial_guess(p, 2.0, {}), torch.tensor([14.0, 18.0], dtype=torch.float64))
+    sim._p_hist = (torch.tensor([8.0, 21.0], dtype=torch.float64), 4.0)
+    # p + (p - p_last) * dt / dt_last with dt / dt_last = 1/2
+    assert torch.equal(sim._initial_guess(p, 2.0, {}), torch.tensor([11.0, 19.5], dtype=torch.float64))
+    # after a shorter step (one the attempt loop cut) the current pressure
+    sim._p_hist = (torch.tensor([8.0, 21.0], dtype=torch.float64), 0.5)
+    assert sim._initial_guess(p, 2.0, {}) is p
     assert sim._initial_guess(p, 2.0, {"initial_guess": "previous"}) is p
     assert torch.equal(sim._initial_guess(p, 2.0, {"initial_guess": "zero"}), torch.zeros(2, dtype=torch.float64))
     sim._p_hist = (torch.zeros(3, dtype=torch.float64), 1.0)  # another grid: no extrapolation

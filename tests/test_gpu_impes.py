@@ -221,6 +221,44 @@ def test_residual_stop_alone_misses_pressure_bar(dev):
         run_parity(cfg, fields, dt, 6, amg=solver_cfg(32, error_tol=0.0))
 
 
+@pytest.mark.parametrize("limit,phase", [("wopr", "oil"), ("wlpr", "water"), ("wgpr", "gas"), ("liqr", "liquid")])
+def test_producer_rate_limits_match_oracle(dev, limit, phase):
+    """Surface-rate producer cut back by each rate limit (simulation.py:1262-1320),
+    three-phase with PVT: the device step against the oracle, wells included."""
+    cfg = {"simulation": {"solver_type": "impes"},
+           "reservoir": {"dimensions": [5, 5, 1], "grid_size": [10, 10, 10], "porosity": 0.25,
+                         "permeability": 200.0, "k_vertical_fraction": 1.0},
+           "fluid": {"pressure": 25.0, "s_w": 0.45, "s_g": 0.15, "pvt_table": json.load(open(PVT_SYNTH))},
+           "wells": [{"name": "P", "type": "producer", "i": 2, "j": 2, "k": 0, "radius": 0.1,
+                      "control_type": "rate", "control_value": 8.0, "rate_type": "surface",
+                      "surface_phase": "liquid", "limits": {limit: 1.0 if phase != "gas" else 40.0}},
+                     {"name": "I", "type": "injector", "i": 0, "j": 0, "k": 0, "radius": 0.1,
+                      "control_type": "rate", "control_value": 5.0, "injected_phase": "water"}]}
+    sim, _ = run_parity(cfg, None, 86400.0, 2, amg=solver_cfg(64))
+    w = sim.well_manager.get_wells()[0]
+    assert w.last_surface_rates[phase] == pytest.approx(1.0 if phase != "gas" else 40.0, rel=1e-9)
+
+
+def test_extrapolated_start_after_a_cut_step(dev):
+    """After a step shorter than the next one (as the attempt loop leaves behind
+    when it cuts dt), the extrapolated start falls back to the current pressure:
+    from identical states the next full step then solves exactly as with
+    initial_guess "previous"."""
+    from simulator.cases import baseline_case
+
+    cfg, fields = baseline_case(1, dims=(20, 20, 6))
+    dt = cfg["simulation"]["time_step_days"] * 86400.0
+    out = {}
+    for mode in ("extrapolate", "previous"):
+        sim = gpu_sim(cfg, fields, pressure_solver="amg", amg=solver_cfg(32, initial_guess="previous"))
+        assert sim.run_step(dt) and sim.run_step(dt / 8)
+        sim._amg_cfg["initial_guess"] = mode
+        assert sim.run_step(dt)
+        out[mode] = (sim.last_solve["iterations"], sim.fluid.pressure.clone())
+    assert out["extrapolate"][0] == out["previous"][0]
+    assert torch.equal(out["extrapolate"][1], out["previous"][1])
+
+
 def test_reference_cg_solver_path(dev):
     """pressure_solver 'cg' (the reference default): Jacobi-PCG with the absolute stop."""
     cfg, dt = golden_config("ow_2d")

@@ -70,8 +70,10 @@ def test_amg_matches_cg_pressure_field():
     assert rel < 5e-4
 
 
-def test_classical_amg_quality_on_laplacian():
-    """tests/test_amg_pressure_solver.py:89-106 through the ClassicalAMG facade."""
+@pytest.mark.parametrize("hierarchy", ["default", "reference"])
+def test_classical_amg_quality_on_laplacian(hierarchy):
+    """tests/test_amg_pressure_solver.py:89-106 through the ClassicalAMG facade,
+    with this package's hierarchy and with the reference's."""
     from solver.classical_amg import ClassicalAMG, _apply_reference_fix
 
     n = 12
@@ -86,7 +88,7 @@ def test_classical_amg_quality_on_laplacian():
     A = torch.sparse_coo_tensor(torch.tensor([rows, cols]), torch.tensor(vals, dtype=torch.float64),
                                 (n * n, n * n)).coalesce().to_sparse_csr()
     A_fixed = _apply_reference_fix(A, anchor_idx=0)
-    amg = ClassicalAMG(A_fixed, theta=0.25, max_levels=10, coarsest_size=40, anchor_idx=None)
+    amg = ClassicalAMG(A_fixed, theta=0.25, max_levels=10, coarsest_size=40, anchor_idx=None, hierarchy=hierarchy)
     N = A_fixed.size(0)
     vs = [torch.ones(N, dtype=torch.float64), torch.linspace(0, 1, N, dtype=torch.float64),
           torch.randn(N, dtype=torch.float64, generator=torch.Generator().manual_seed(0))]
@@ -123,6 +125,73 @@ def test_amg_solve_and_preconditioner_facades():
     r = torch.tensor(np.random.default_rng(1).standard_normal(A.shape[0]), device=pre["device"])
     z = pre["apply"](r)
     assert torch.isfinite(z).all() and z.shape == r.shape
+
+
+@pytest.mark.parametrize("hierarchy", ["default", "reference"])
+def test_classical_amg_levels_metadata(hierarchy):
+    """ClassicalAMG.levels carries the reference's keys (classical_amg.py:891-904,
+    :925-933), each checked against its definition recomputed on the host from
+    the exported A, P and C/F markers."""
+    from matrices import impes_like
+    from solver.classical_amg import ClassicalAMG
+
+    A, *_ = impes_like(16, 16, 6, seed=5)
+    At = torch.sparse_csr_tensor(torch.tensor(A.indptr), torch.tensor(A.indices), torch.tensor(A.data),
+                                 size=A.shape)
+    amg = ClassicalAMG(At, coarsest_size=50, hierarchy=hierarchy)
+    lv = amg.levels
+    assert len(lv) >= 3 and lv[-1]["is_coarsest"] and not any(d["is_coarsest"] for d in lv[:-1])
+    coords = np.arange(A.shape[0], dtype=np.float64)
+    for d in lv:
+        Al = d["A"].to_dense().cpu().numpy()
+        assert d["n"] == Al.shape[0] and d["basis"] is None
+        assert np.array_equal(d["diag"].cpu().numpy(), np.abs(np.diag(Al)))
+        if d["is_coarsest"]:
+            assert d["P"] is None
+            continue
+        P = d["P"].to_dense().cpu().numpy()
+        cf = d["cf_marker"].cpu().numpy()
+        c, f = np.flatnonzero(cf == 0), np.flatnonzero(cf != 0)
+        assert np.array_equal(d["coarse_nodes"].cpu().numpy(), c) and np.array_equal(d["F_nodes"].cpu().numpy(), f)
+        f2c = d["fine2coarse"].cpu().numpy()
+        assert np.array_equal(f2c[c], np.arange(c.size)) and np.all(f2c[f] == -1)
+        assert np.array_equal(P[c], np.eye(P.shape[1])[f2c[c]])  # C rows inject
+        assert np.array_equal(d["node_coords"].cpu().numpy(), coords)
+        m = d["interp_metrics"]
+        rs = P.sum(axis=1)
+        assert np.isclose(m["row_err_max"], np.max(np.abs(rs - 1)), rtol=1e-12, atol=1e-15)
+        assert np.isclose(m["const_rel"], np.linalg.norm(rs - 1) / np.sqrt(rs.size), rtol=1e-12, atol=1e-15)
+        lin = P @ coords[c] - coords
+        assert np.isclose(m["grad_rel"], np.linalg.norm(lin) / np.linalg.norm(coords), rtol=1e-12)
+        w = d["P"].values().cpu().numpy()
+        assert m["neg_count"] == int(np.sum(w < -1e-12)) and m["pos_count"] == int(np.sum(w > 1e-12))
+        assert m["min_w"] == w.min() and m["max_w"] == w.max()
+        coords = coords[c]
+
+
+def test_amg_solve_reference_hierarchy():
+    """amg_solve with the reference's hierarchy choices converges to the same
+    answer as with the default one (they change iteration counts only), each
+    cached under its own key."""
+    import scipy.sparse.linalg as spl
+
+    from matrices import impes_like
+    from solver import pressure_amg as pa
+
+    A, *_ = impes_like(20, 20, 5, seed=4)
+    At = torch.sparse_csr_tensor(torch.tensor(A.indptr), torch.tensor(A.indices), torch.tensor(A.data),
+                                 size=A.shape)
+    b = torch.tensor(A @ (2e7 + 1e5 * np.random.default_rng(2).standard_normal(A.shape[0])))
+    xs = spl.spsolve(A.tocsc(), b.numpy())
+    its = {}
+    for mode in ("default", "reference"):
+        x = pa.amg_solve(At, b, tol=1e-10, hierarchy=mode, maxiter=600)
+        assert pa.amg_solve.last_stats["converged"], mode
+        its[mode] = pa.amg_solve.last_stats["iterations"]
+        assert np.linalg.norm(x.numpy() - xs) / np.linalg.norm(xs) < 1e-7, mode
+    assert sum(1 for k in pa._AMG_CACHE if k[0] == tuple(A.shape) and k[-1] in ("default", "reference")) >= 2
+    with pytest.raises(ValueError):
+        pa.amg_solve(At, b, hierarchy="other")
 
 
 def test_impes_3phase_smoke():

@@ -220,3 +220,51 @@ def test_saturation_bounds_mild_case():
     assert np.all(s["sg"] >= c.sg_cr) and np.all(s["sg"] <= 1 - c.so_r - s["sw"] + 1e-8)
     err = sim.volume_balance_error()
     assert err["water"] < 1e-12 and err["gas"] < 1e-12, err
+
+
+@pytest.mark.parametrize("limit,phase", [("wopr", "oil"), ("wlpr", "water"), ("wgpr", "gas"), ("liqr", "liquid")])
+def test_producer_rate_limits_cap_surface_rates(limit, phase):
+    """simulation.py:1262-1320: a surface-rate producer whose control exceeds a
+    limit is cut back so that the limited surface rate equals its cap while the
+    other surface rates follow in proportion (same cell fractions); without the
+    limit the same well produces its control. Surface rates are recomputed here
+    from the oracle's reservoir rate and the cell's fractional flows and PVT
+    values (the definitions of well.last_surface_rates). The cap is half the
+    unlimited well's surface rate of that phase."""
+    base = {"simulation": {"solver_type": "impes"},
+            "reservoir": {"dimensions": [5, 5, 1], "grid_size": [10, 10, 10], "porosity": 0.25,
+                          "permeability": 200.0, "k_vertical_fraction": 1.0},
+            "fluid": {"pressure": 25.0, "s_w": 0.45, "s_g": 0.15, "pvt_table": PVT_SYNTH},
+            "wells": [{"name": "P", "type": "producer", "i": 2, "j": 2, "k": 0, "radius": 0.1,
+                       "control_type": "rate", "control_value": 8.0, "rate_type": "surface",
+                       "surface_phase": "liquid"}]}
+    out = {}
+    cap = None
+    for limited in (False, True):
+        cfg = json.loads(json.dumps(base))
+        if limited:
+            cap = 0.5 * out[False][0]["sat"]["wells"][0]["last_surface_rates"][phase]
+            cfg["wells"][0]["limits"] = {limit: cap}
+        case = oi.Case(cfg, pvt=oi.PVT(PVT_SYNTH))
+        sim = oi.Simulation(case)
+        st = dict(sim.state)
+        rec = sim.step(86400.0)
+        out[limited] = (rec, st, case)
+    free, lim = out[False][0]["sat"]["wells"][0], out[True][0]["sat"]["wells"][0]
+    rec, st, case = out[True]
+    p = rec["p"][2, 2, 0]
+    kro, krw, krg = case.relperms(st["sw"][2, 2, 0], st["sg"][2, 2, 0])
+    mu_o, mu_w, mu_g = (float(case.pvt(k, p)) * 1e-3 for k in ("mu_o", "mu_w", "mu_g"))
+    mo, mw, mg = kro / mu_o, krw / mu_w, krg / mu_g
+    mt = mo + mw + mg + 1e-10
+    fo, fw, fg = mo / mt, mw / mt, mg / mt
+    bo, bw, bg, rs, rv = (float(case.pvt(k, p)) for k in ("Bo", "Bw", "Bg", "Rs", "Rv"))
+    q = abs(lim["last_q_total"])
+    surf = {"oil": q * (fo / bo + fg * rv / bg) * 86400.0, "water": q * fw / bw * 86400.0,
+            "gas": q * (fg / bg + fo * rs / bo) * 86400.0}
+    surf["liquid"] = surf["oil"] + surf["water"]
+    for k, v in surf.items():
+        assert math.isclose(lim["last_surface_rates"][k], v, rel_tol=1e-12), k
+    assert math.isclose(surf[phase], cap, rel_tol=1e-12)
+    # the unlimited well produced its liquid control (no choke binds)
+    assert math.isclose(free["last_surface_rates"]["liquid"], 8.0, rel_tol=1e-9)
