@@ -187,6 +187,18 @@ int impes_fgmres_st7(impes_solver* solver, impes_amg* amg, int precision, int nx
 int impes_amg_cycles(impes_amg* amg, int precision, const double* b, double* x, int use_x0, int min_cycles,
                      double tol, int max_cycles, double* rel, int* cycles_done, void* stream);
 
+/* The same iteration as impes_amg_cycles with no device allocation: its workspace
+ * (three FP64 n-vectors and the reduction partials) is a static device array of the
+ * library, 24 MiB per GPU, held from library load to process exit. No counterpart in
+ * the reference; it is for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc). Serves hierarchies of up to 1,048,576 rows and
+ * fails with an error above that. All calls on one GPU share the workspace: calls
+ * that overlap on one GPU (other threads, other streams) race. Nothing in the
+ * workspace carries over from one call to the next. */
+int impes_amg_cycles_noalloc(impes_amg* amg, int precision, const double* b, double* x, int use_x0,
+                             int min_cycles, double tol, int max_cycles, double* rel, int* cycles_done,
+                             void* stream);
+
 /* ------------------------------------------------------------------ IMPES model */
 
 typedef struct {

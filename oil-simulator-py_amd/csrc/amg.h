@@ -142,6 +142,18 @@ class Hierarchy {
   // least min_cycles, or max_cycles; x = dh.*xs. Returns the cycles done, *rel the residual.
   int cycles(int precision, const double* b, double* x, bool use_x0, int min_cycles, double tol, int max_cycles,
              double* rel, hipStream_t s);
+  // Workspace of cycles(): three n0-vectors (scaled b, scaled x, residual; each
+  // padded to 32 doubles), the
+  // kRedBlocks reduction partials and 4 scalars. Nothing in it needs clearing: every
+  // entry is written before it is read.
+  struct CycleWork {
+    double *b = nullptr, *x = nullptr, *r = nullptr, *part = nullptr, *sc = nullptr;
+  };
+  static long long cycle_work_doubles(long long n) { return 3 * ((n + 31) / 32 * 32) + kRedBlocks + 4; }
+  // The same iteration in a caller-provided workspace of cycle_work_doubles(n0)
+  // doubles at `ws` (256-byte aligned): no device allocation.
+  int cycles_in(double* ws, int precision, const double* b, double* x, bool use_x0, int min_cycles, double tol,
+                int max_cycles, double* rel, hipStream_t s);
 
   int n0() const { return levels_.empty() ? 0 : levels_[0].n; }
   int num_levels() const { return (int)levels_.size(); }
@@ -178,6 +190,8 @@ class Hierarchy {
   void coarse_solve(int l, const T* b, T* x, hipStream_t s);
   void drop_graphs();
   void capture_vcycle(int precision);
+  int cycles_impl(const CycleWork& w, int precision, const double* b, double* x, bool use_x0, int min_cycles,
+                  double tol, int max_cycles, double* rel, hipStream_t s);
 
   bool graphs_ = true;
   bool transposed_ = false;  // eager cycles of vcycle64/32 with nu1 and nu2 swapped

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -835,17 +836,33 @@ void Hierarchy::vcycle32(const float* b, float* x, bool x_zero, hipStream_t s) {
 int Hierarchy::cycles(int precision, const double* b, double* x, bool use_x0, int min_cycles, double tol,
                       int max_cycles, double* rel, hipStream_t s) {
   IMPES_REQUIRE(!levels_.empty(), "cycles before setup");
-  Level& L0 = levels_[0];
-  const long long n = L0.n;
+  const long long n = levels_[0].n;
   cyc_b_.reserve(n); cyc_x_.reserve(n); cyc_r_.reserve(n);
   cyc_s_.reserve(kRedBlocks + 4);
-  double* work = cyc_s_.p;
-  double* sc = cyc_s_.p + kRedBlocks;
+  CycleWork w{cyc_b_.p, cyc_x_.p, cyc_r_.p, cyc_s_.p, cyc_s_.p + kRedBlocks};
+  return cycles_impl(w, precision, b, x, use_x0, min_cycles, tol, max_cycles, rel, s);
+}
+
+int Hierarchy::cycles_in(double* ws, int precision, const double* b, double* x, bool use_x0, int min_cycles,
+                         double tol, int max_cycles, double* rel, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "cycles before setup");
+  IMPES_REQUIRE(ws && (reinterpret_cast<uintptr_t>(ws) & 255) == 0, "cycles: workspace must be 256-byte aligned");
+  const long long n = levels_[0].n;
+  // the n-vectors start on 256-byte boundaries (n rounded up to 32 doubles)
+  const long long ld = (n + 31) / 32 * 32;
+  CycleWork w{ws, ws + ld, ws + 2 * ld, ws + 3 * ld, ws + 3 * ld + kRedBlocks};
+  return cycles_impl(w, precision, b, x, use_x0, min_cycles, tol, max_cycles, rel, s);
+}
+
+int Hierarchy::cycles_impl(const CycleWork& w, int precision, const double* b, double* x, bool use_x0,
+                           int min_cycles, double tol, int max_cycles, double* rel, hipStream_t s) {
+  Level& L0 = levels_[0];
+  const long long n = L0.n;
   double hv[2];
-  launch_scale_f64(cyc_b_.p, dh_.p, b, n, s);  // bs = dh .* b
-  if (use_x0) launch_div_f64(cyc_x_.p, x, dh_.p, n, s);  // xs = x0 ./ dh
-  launch_dot_f64(cyc_b_.p, cyc_b_.p, n, sc, work, s);
-  IMPES_HIP_CHECK(hipMemcpyAsync(hv, sc, sizeof(double), hipMemcpyDeviceToHost, s));
+  launch_scale_f64(w.b, dh_.p, b, n, s);  // bs = dh .* b
+  if (use_x0) launch_div_f64(w.x, x, dh_.p, n, s);  // xs = x0 ./ dh
+  launch_dot_f64(w.b, w.b, n, w.sc, w.part, s);
+  IMPES_HIP_CHECK(hipMemcpyAsync(hv, w.sc, sizeof(double), hipMemcpyDeviceToHost, s));
   IMPES_HIP_CHECK(hipStreamSynchronize(s));
   const double bnorm = std::sqrt(hv[0]);
   bool zero = !use_x0;
@@ -853,32 +870,28 @@ int Hierarchy::cycles(int precision, const double* b, double* x, bool use_x0, in
   double r = INFINITY;
   while (c < max_cycles) {
     if (precision == 32) {
-      launch_f64_to_f32(L0.b32.p, cyc_b_.p, n, s);
-      if (!zero) launch_f64_to_f32(L0.x32.p, cyc_x_.p, n, s);
+      launch_f64_to_f32(L0.b32.p, w.b, n, s);
+      if (!zero) launch_f64_to_f32(L0.x32.p, w.x, n, s);
       vcycle_level<float>(0, L0.b32.p, L0.x32.p, zero, s);
-      DevBuf<double> ones;
-      ones.alloc(n);
-      launch_fill_f64(ones.p, 1.0, n, s);
-      launch_scale_f32_to_f64(cyc_x_.p, ones.p, L0.x32.p, n, s);
-      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      launch_f32_to_f64(w.x, L0.x32.p, n, s);
     } else {
-      vcycle_level<double>(0, cyc_b_.p, cyc_x_.p, zero, s);
+      vcycle_level<double>(0, w.b, w.x, zero, s);
     }
     zero = false;
     ++c;
     if (stencil_) {
       Stencil7<double> S{nx_, ny_, nz_, sd_.p, sx_.p, sy_.p, sz_.p};
-      launch_st7_residual_f64(S, cyc_x_.p, cyc_b_.p, cyc_r_.p, s);
+      launch_st7_residual_f64(S, w.x, w.b, w.r, s);
     } else {
-      launch_residual_f64(Sel<double>::A(L0), cyc_x_.p, cyc_b_.p, cyc_r_.p, L0.group, s);
+      launch_residual_f64(Sel<double>::A(L0), w.x, w.b, w.r, L0.group, s);
     }
-    launch_dot_f64(cyc_r_.p, cyc_r_.p, n, sc + 1, work, s);
-    IMPES_HIP_CHECK(hipMemcpyAsync(hv + 1, sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    launch_dot_f64(w.r, w.r, n, w.sc + 1, w.part, s);
+    IMPES_HIP_CHECK(hipMemcpyAsync(hv + 1, w.sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
     IMPES_HIP_CHECK(hipStreamSynchronize(s));
     r = std::sqrt(hv[1]) / (bnorm + 1e-30);
     if (c >= min_cycles && r < tol) break;
   }
-  launch_scale_f64(x, dh_.p, cyc_x_.p, n, s);
+  launch_scale_f64(x, dh_.p, w.x, n, s);
   IMPES_HIP_CHECK(hipStreamSynchronize(s));
   if (rel) *rel = r;
   return c;

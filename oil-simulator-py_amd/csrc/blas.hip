@@ -155,6 +155,12 @@ void launch_copy_f32(float* y, const float* x, long long n, hipStream_t s) {
   k_copy<float, float><<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, n);
   IMPES_LAUNCH_CHECK();
 }
+void launch_f32_to_f64(double* y, const float* x, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_copy<double, float><<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, n);
+  IMPES_LAUNCH_CHECK();
+}
+
 void launch_f64_to_f32(float* y, const double* x, long long n, hipStream_t s) {
   if (n <= 0) return;
   k_copy<float, double><<<grid_for(n, kBlock), kBlock, 0, s>>>(y, x, n);

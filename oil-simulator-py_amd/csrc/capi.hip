@@ -1,6 +1,7 @@
 // extern "C" entry points: argument checking, exception -> error code mapping.
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,38 @@ struct impes_solver {
 };
 
 static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device workspace of impes_amg_cycles_noalloc: a file-scope array that every GPU
+// the library runs on holds from module load until the process ends. It serves
+// hierarchies of up to kNoAllocMaxRows rows (3 padded FP64 vectors + reduction
+// partials: 24 MiB + 4 KiB). Host code never passes `scratch` itself to a kernel
+// (in host code it names a host-side stand-in): it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device.
+constexpr long long kNoAllocMaxRows = 1LL << 20;
+constexpr long long kNoAllocScratch = 3 * kNoAllocMaxRows + kRedBlocks + 64;
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up on the first call
+// made with that device current)
+double* scratch_on_current_device() {
+  int dev = -1;
+  IMPES_HIP_CHECK(hipGetDevice(&dev));
+  IMPES_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    IMPES_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    IMPES_REQUIRE(p != nullptr, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 // impes_amg_params -> AmgParams, with range checks shared by every entry point
 static AmgParams params_from_c(const impes_amg_params* c) {
@@ -429,6 +462,26 @@ int impes_amg_cycles(impes_amg* amg, int precision, const double* b, double* x, 
     IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
     IMPES_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
     const int c = amg->h.cycles(precision, b, x, use_x0 != 0, min_cycles, tol, max_cycles, rel, as_stream(stream));
+    if (cycles_done) *cycles_done = c;
+  });
+}
+
+int impes_amg_cycles_noalloc(impes_amg* amg, int precision, const double* b, double* x, int use_x0,
+                             int min_cycles, double tol, int max_cycles, double* rel, int* cycles_done,
+                             void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(amg && b && x, "null pointer");
+    IMPES_REQUIRE(amg->h.num_levels() > 0, "not set up");
+    IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    IMPES_REQUIRE(max_cycles >= 1, "max_cycles must be >= 1");
+    const long long n = amg->h.n0();
+    IMPES_REQUIRE(n <= kNoAllocMaxRows && Hierarchy::cycle_work_doubles(n) <= kNoAllocScratch,
+                  "impes_amg_cycles_noalloc: the hierarchy has " + std::to_string(n) +
+                      " rows, above the static workspace's maximum of " + std::to_string(kNoAllocMaxRows) +
+                      " (use impes_amg_cycles)");
+    double* ws = scratch_on_current_device();
+    const int c =
+        amg->h.cycles_in(ws, precision, b, x, use_x0 != 0, min_cycles, tol, max_cycles, rel, as_stream(stream));
     if (cycles_done) *cycles_done = c;
   });
 }

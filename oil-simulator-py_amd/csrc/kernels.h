@@ -17,6 +17,7 @@ void launch_fill_i32(int* x, int v, long long n, hipStream_t s);
 void launch_copy_f64(double* y, const double* x, long long n, hipStream_t s);
 void launch_copy_f32(float* y, const float* x, long long n, hipStream_t s);
 void launch_f64_to_f32(float* y, const double* x, long long n, hipStream_t s);
+void launch_f32_to_f64(double* y, const float* x, long long n, hipStream_t s);
 // y = a .* x (elementwise), converting precision as needed
 void launch_scale_f64_to_f32(float* y, const double* a, const double* x, long long n, hipStream_t s);
 void launch_scale_f32_to_f64(double* y, const double* a, const float* x, long long n, hipStream_t s);

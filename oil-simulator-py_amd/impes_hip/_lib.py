@@ -135,6 +135,7 @@ SIGNATURES = {
     "impes_fgmres_st7": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _I, _I,
                          C.POINTER(SolveStats), _P],
     "impes_amg_cycles": [_P, _I, _P, _P, _I, _I, _D, _I, _PD, _PI, _P],
+    "impes_amg_cycles_noalloc": [_P, _I, _P, _P, _I, _I, _D, _I, _PD, _PI, _P],
     "impes_model_create": [C.POINTER(Grid), C.POINTER(FluidStruct), _I, _PD, _I, C.POINTER(WellStruct),
                            C.POINTER(_P)],
     "impes_model_destroy": [_P],
