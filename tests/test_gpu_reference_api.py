@@ -379,6 +379,61 @@ def test_run_writes_the_reference_artifacts(tmp_path):
     assert f"total net={net:.6e}" in txt
 
 
+def test_run_artifacts_match_oracle_run(tmp_path):
+    """run() (simulation.py:2197-2248) on the gas3 case, 4 daily steps with the
+    benchmark's solver settings, against an oracle run of the same case (exact
+    pressure solves, its own state propagated): the final and intermediate npz fields (pressure 1e-7 relative,
+    saturations 1e-6 absolute) and the results.txt figures (summary lines
+    identical, mass-balance in/out/net and total 1e-6 relative against the
+    oracle's cumulative mass ledger)."""
+    import re
+
+    from oracle import impes as oi
+
+    from simulator.cases import BENCH_AMG
+
+    cfg = gas3_config()
+    cfg["simulation"].update({"solver_type": "impes", "total_time_days": 4, "time_step_days": 1.0,
+                              "save_interval": 2, "pressure_solver": "amg", "amg": dict(BENCH_AMG)})
+    _, _, fl, sim = mk(cfg)
+    out = sim.run("case", results_root=str(tmp_path))
+    orc = oi.Simulation(oi.Case(cfg))
+    states = []
+    for _ in range(4):
+        orc.step(86400.0)
+        st = orc.state
+        states.append({"pressure": st["p"], "sw": st["sw"], "sg": st["sg"], "so": 1.0 - st["sw"] - st["sg"]})
+
+    def check_fields(z, ref):
+        assert np.max(np.abs(z["pressure"] - ref["pressure"])) <= 1e-7 * np.max(np.abs(ref["pressure"]))
+        for k in ("sw", "sg", "so"):
+            assert np.max(np.abs(z[k] - ref[k])) <= 1e-6, k
+
+    check_fields(np.load(os.path.join(out, "case.npz")), states[-1])
+    for step in (2, 4):
+        check_fields(np.load(os.path.join(out, "intermediate", f"case_step_{step}.npz")), states[step - 1])
+    txt = open(os.path.join(out, "case.txt")).read()
+    p, sw, sg, so = (states[-1][k] for k in ("pressure", "sw", "sg", "so"))
+    expect = [f"Pressure MPa: min={p.min()/1e6:.2f}, max={p.max()/1e6:.2f}, mean={p.mean()/1e6:.2f}",
+              f"Sw: min={sw.min():.4f}, max={sw.max():.4f}, mean={sw.mean():.4f}",
+              f"Sg: min={sg.min():.4f}, max={sg.max():.4f}, mean={sg.mean():.4f}",
+              f"So: min={so.min():.4f}, max={so.max():.4f}, mean={so.mean():.4f}"]
+    for line in expect:
+        assert line in txt, line
+    mb = orc.mass_balance
+    total = 0.0
+    for ph in ("water", "gas", "oil"):
+        m = re.search(rf"^{ph}:\s+in=(\S+), out=(\S+), net=(\S+)$", txt, re.M)
+        vin, vout, net = (float(v) for v in m.groups())
+        tol = 1e-6 * (abs(mb[ph]["in"]) + abs(mb[ph]["out"])) + 1e-9
+        assert abs(vin - mb[ph]["in"]) <= tol and abs(vout - mb[ph]["out"]) <= tol, ph
+        assert abs(net - (mb[ph]["in"] - mb[ph]["out"])) <= tol, ph
+        total += mb[ph]["in"] - mb[ph]["out"]
+    m = re.search(r"^total net=(\S+)$", txt, re.M)
+    scale = sum(abs(mb[ph]["in"]) + abs(mb[ph]["out"]) for ph in mb)
+    assert abs(float(m.group(1)) - total) <= 1e-6 * scale
+
+
 def test_update_matrix_rejects_a_size_change():
     """ClassicalAMG.update_matrix raises ValueError when the new matrix has
     another size (reference classical_amg.py:1366-1367) and keeps working for

@@ -102,3 +102,66 @@ def test_batched_env_matches_single_envs(tmp_path, dev):
     # the copies really evolved apart (different actions)
     p = benv.fluid.pressure.view(B, -1)
     assert (p[0] - p[1]).abs().max().item() > 1e3
+
+
+class OracleEnv:
+    """ReservoirEnv (rl_env.py:90-154) restated on the FP64 oracle with the exact
+    pressure solve: actions move the oracle wells' controls by the same rule
+    (rl_env.py:129-140), the observation is (p / 1e7, Sw) in float32
+    (rl_env.py:123-127) and the reward the oil mass produced in the step / 1e6
+    (rl_env.py:142-154: porosity * So * rho_o(p) * cell volume summed)."""
+
+    def __init__(self, cfg):
+        from oracle import impes as oi
+
+        self.oi = oi
+        pvt = oi.PVT(cfg["fluid"]["pvt_table"]) if cfg["fluid"].get("pvt_table") else None
+        self.case = oi.Case(cfg, pvt=pvt)
+        self.sim = oi.Simulation(self.case)
+        self.dt = float(cfg["simulation"].get("time_step_days", 1.0)) * 86400.0
+        self.mass = self.oil_mass()
+
+    def oil_mass(self):
+        st = self.sim.state
+        so = 1.0 - st["sw"] - st["sg"]
+        return float(np.sum(self.case.pv * so * self.case.densities(st["p"])[1]))
+
+    def obs(self):
+        st = self.sim.state
+        return np.concatenate([st["p"].ravel() / 1e7, st["sw"].ravel()]).astype(np.float32)
+
+    def step(self, action):
+        for a, w in zip(action, self.case.wells):
+            if w.control_type == "rate":
+                w.control_value = float(w.control_value + w.control_value * 0.2 * float(a))
+            elif w.control_type == "bhp":
+                w.control_value = float(w.control_value + 5.0 * float(a))
+        self.sim.step(self.dt)
+        m = self.oil_mass()
+        r = (self.mass - m) / 1e6
+        self.mass = m
+        return self.obs(), r
+
+
+def test_env_trajectory_matches_oracle(tmp_path, dev):
+    """A ReservoirEnv trajectory under a fixed random action sequence against the
+    oracle-stepped environment with the same actions (each side propagates its own
+    state): observations, rewards, well controls and substep counts at every step."""
+    from rl_env import ReservoirEnv
+
+    path, cfg = _cfg_path(tmp_path)
+    steps = 6
+    actions = np.random.default_rng(11).uniform(-1.0, 1.0, size=(steps, N_WELLS))
+    env = ReservoirEnv(path, max_steps=steps, device=dev)
+    obs, _ = env.reset(seed=0)
+    orc = OracleEnv(cfg)
+    assert np.array_equal(obs, orc.obs())
+    for t in range(steps):
+        o, r, term, trunc, info = env.step(actions[t])
+        oo, ro = orc.step(actions[t])
+        assert info["converged"]
+        assert [w.control_value for w in env.well_manager.wells] == [w.control_value for w in orc.case.wells]
+        assert env.simulator.last_substeps == orc.sim.history[-1]["N"], t
+        assert np.allclose(o, oo, rtol=0, atol=2e-6), (t, np.max(np.abs(o - oo)))
+        assert r == pytest.approx(ro, rel=1e-6, abs=1e-12), (t, r, ro)
+    assert term
