@@ -281,6 +281,44 @@ int impes_model_solve_cg(impes_model* m, double tol, int maxit, double* p_new, i
 int impes_model_saturation(impes_model* m, double dt, const double* p_prev, const double* p_new, const double* sw,
                            const double* sg, double* sw_out, double* sg_out, double* so_out, double* report_host,
                            void* stream);
+/* ---- reverse sweep of the step (the hand-written adjoint). No counterpart in the
+ * reference (its ROADMAP.md:53 lists an adjoint as future work). One reverse step n,
+ * for the step from (p_prev, sw, sg) to (p_new, sw_next, sg_next):
+ *   impes_model_assemble(m, dt, p_prev, sw, sg)       re-assemble step n;
+ *   impes_model_transport_vjp(...)                    transport reversed;
+ *   impes_model_solve_adjoint(m, ..., g_p_new, lam)   lam = A^-1 g_p_new (A = A^T);
+ *   impes_model_assemble_vjp(..., lam, ...)           assembly reversed.
+ * Cotangents of the face transmissibilities and well controls accumulate in the
+ * model from impes_model_adjoint_reset on; impes_model_rock_vjp turns the former into
+ * permeability cotangents, impes_model_adjoint_values returns the latter.
+ * Branches (upwinding, substep count, clamps, chokes, limits, bhp_min) are those of
+ * the forward values. Batched environments are not supported. */
+int impes_model_adjoint_reset(impes_model* m, void* stream);
+/* Replaces the transport part of the reverse sweep that the reference lacks for
+ * Simulator._impes_saturation_step (simulation.py:968-1544). Inputs: g_sw_next,
+ * g_sg_next (cotangents of the step's output saturations), g_obs_host (host,
+ * nwells x 3: cotangents of the producers' surface oil/water/gas rates, the
+ * report's observation slots). Writes g_sw, g_sg (cotangents of sw, sg); adds into
+ * g_p_new (cotangent of p_new) and g_p_prev. Synchronises the stream. */
+int impes_model_transport_vjp(impes_model* m, double dt, const double* p_prev, const double* p_new,
+                              const double* sw, const double* sg, const double* g_sw_next, const double* g_sg_next,
+                              const double* g_obs_host, double* g_p_new, double* g_p_prev, double* g_sw, double* g_sg,
+                              void* stream);
+/* Transposed pressure solve of the assembled step (the operator is symmetric: the
+ * forward's hierarchy, numerically updated, inside FP64 FGMRES from a zero start). */
+int impes_model_solve_adjoint(impes_model* m, int precision, double rtol, int restart, int maxit, const double* g,
+                              double* lam, impes_solve_stats* stats, void* stream);
+/* Reverse of the pressure assembly (simulation.py:867-902, 1546-1660) given the
+ * adjoint solution lam and the solved p_new: adds into g_p_prev, g_sw, g_sg. */
+int impes_model_assemble_vjp(impes_model* m, double dt, const double* p_prev, const double* sw, const double* sg,
+                             const double* p_new, const double* lam, double* g_p_prev, double* g_sw, double* g_sg,
+                             void* stream);
+/* Reverse of the transmissibilities (simulation.py:52-74): accumulated face
+ * cotangents -> cotangents of kx, ky, kz (m^2) per cell. */
+int impes_model_rock_vjp(impes_model* m, const double* kx, const double* ky, const double* kz, double* g_kx,
+                         double* g_ky, double* g_kz, void* stream);
+/* Accumulated cotangents of the well control values (host, nwells). */
+int impes_model_adjoint_values(impes_model* m, double* g_values_host, void* stream);
 /* Copies a model field into out (device, n cells): 0 diag, 1 ox, 2 oy, 3 oz, 4 rhs, 5 accumulation,
  * 6 gravity+capillary rhs, 7 tx, 8 ty, 9 tz, 10 pore volume, 11 total mobility, 12 1/diag. */
 int impes_model_export(impes_model* m, int field, double* out, void* stream);

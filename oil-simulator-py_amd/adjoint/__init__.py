@@ -1,7 +1,9 @@
 """Discrete adjoints of the IMPES step: hand-written reverse sweep for two-phase
-oil-water cases (``twophase``) and the three-phase black-oil step with torch
-autograd around the HIP pressure solve (``threephase``)."""
-from . import threephase  # noqa: F401
+oil-water cases (``twophase``), the three-phase black-oil step on the HIP
+kernels with the hand-written reverse-sweep kernels of csrc/adjoint.hip
+(``device``), and its torch restatement differentiated by autograd
+(``threephase``, the reference the device adjoint is checked against)."""
+from . import device, threephase  # noqa: F401
 from .twophase import (  # noqa: F401
     HistoryMatchMisfit,
     TwoPhaseCase,

@@ -422,8 +422,12 @@ def _clamp01(alpha, branches=None):
     return alpha
 
 
-def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_old, ps, branches=None):
-    """simulation.py:968-1544 (row-balance form of the BHP rate): new saturations."""
+def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_old, ps, branches=None, obs=None):
+    """simulation.py:968-1544 (row-balance form of the BHP rate): new saturations.
+    ``obs`` (a list) receives the step's observations: a (wells, 3) tensor of the
+    surface oil, water and gas rates [m3/day] of every producer (rate and BHP
+    controlled; formation volume factors 1 without a PVT table), zero rows for
+    injectors."""
     f = case.fluid
     Tx, Ty, Tz = T
     P = p_new
@@ -469,7 +473,21 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
     pvt = case.pvt
     so_r, sw_cr, sg_cr = f["so_r"], f["sw_cr"], f["sg_cr"]
     one = torch.ones((), dtype=P.dtype, device=P.device)
+    zero1 = torch.zeros((), dtype=P.dtype, device=P.device)
+    rows = []
+
+    def surface_rates(q, c):
+        """|q| [m3/s reservoir] at cell c -> surface oil, water, gas [m3/day]."""
+        qa = torch.abs(q)
+        fo_, fw_, fg_ = fof[c], fwf[c], fgf[c]
+        if pvt is None:
+            return torch.stack([qa * fo_ * DAY, qa * fw_ * DAY, qa * fg_ * DAY])
+        bo, bw, bg, rs, rv = (pvt(k, Pf[c]) for k in ("Bo", "Bw", "Bg", "Rs", "Rv"))
+        return torch.stack([qa * (fo_ / _mx(bo) + fg_ * rv / _mx(bg)) * DAY, qa * (fw_ / _mx(bw)) * DAY,
+                            qa * (fg_ / _mx(bg) + fo_ * rs / _mx(bo)) * DAY])
+
     for wn, w in enumerate(case.wells):
+        rows.append(torch.stack([zero1, zero1, zero1]))
         c = w["cell"]
         idx = torch.tensor([c], device=P.device)
         add = lambda arr, v: arr.index_add(0, idx, v.reshape(1))  # noqa: E731
@@ -533,6 +551,7 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                                               ("limit", wn))
                             if limited:
                                 q_total = -m
+                rows[wn] = surface_rates(q_total, c)
                 q_w = add(q_w, q_total * fw_)
                 q_g = add(q_g, q_total * fg_)
         elif w["control"] == "bhp":
@@ -546,8 +565,11 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
                 else:
                     q_w = add(q_w, -q_total)
             else:
+                rows[wn] = surface_rates(q_total, c)
                 q_w = add(q_w, -q_total * fwf[c])
                 q_g = add(q_g, -q_total * fgf[c])
+    if obs is not None:
+        obs.append(torch.stack(rows) if rows else torch.zeros((0, 3), dtype=P.dtype, device=P.device))
     dSw = (case.dt / case.pv) * (q_w.reshape(case.dims) - div_w)
     dSg = (case.dt / case.pv) * (q_g.reshape(case.dims) - div_g)
     with torch.no_grad():  # the substep count (a ceil) is taken from the forward values
@@ -583,16 +605,17 @@ def saturation_step(case: ThreePhaseCase, T, values, p_prev, p_new, sw_old, sg_o
     return sw, sg, N
 
 
-def step(case: ThreePhaseCase, lnk, values, p, sw, sg, solve, direct: bool = False, branches=None):
+def step(case: ThreePhaseCase, lnk, values, p, sw, sg, solve, direct: bool = False, branches=None, obs=None):
     """One IMPES step; ``direct``: solve is itself differentiable (dense reference);
-    ``branches``: a list that receives the record of every piecewise decision."""
+    ``branches``: a list that receives the record of every piecewise decision;
+    ``obs``: a list that receives the step's producer surface rates."""
     T = transmissibilities(case, lnk)
     ps = pressure_system(case, T, values, p, sw, sg, branches)
     if direct:
         p_new = solve(ps["diag"], ps["ox"], ps["oy"], ps["oz"], ps["b"])
     else:
         p_new = FaceSolve.apply(ps["diag"], ps["ox"], ps["oy"], ps["oz"], ps["b"], solve)
-    sw_new, sg_new, N = saturation_step(case, T, values, p, p_new, sw, sg, ps, branches)
+    sw_new, sg_new, N = saturation_step(case, T, values, p, p_new, sw, sg, ps, branches, obs)
     return p_new, sw_new, sg_new, N
 
 
@@ -640,3 +663,51 @@ def gradient(case: ThreePhaseCase, lnk, values, solve, steps, checkpoint_every=5
     J = oil_in_place(case, p, sw, sg)
     g_lnk, g_values = torch.autograd.grad(J, (lnk, values))
     return J.detach(), g_lnk, g_values, counts
+
+
+# ---------------------------------------------------------------- history match
+def observe(case: ThreePhaseCase, lnk, values, solve, steps, direct=False, branches=None):
+    """Producer surface rates of every step: (steps, wells, 3) [m3/day]."""
+    p, sw, sg = case.initial_state()
+    out = []
+    for _ in range(steps):
+        p, sw, sg, _ = step(case, lnk, values, p, sw, sg, solve, direct, branches, out)
+    return torch.stack(out)
+
+
+def rate_misfit(d, data, sigma):
+    """0.5 sum ((d - data) / sigma)^2 over steps, producers and phases."""
+    return 0.5 * (((d - data) / sigma) ** 2).sum()
+
+
+def misfit_gradient(case: ThreePhaseCase, lnk, values, solve, steps, data, sigma, checkpoint_every=4,
+                    direct=False):
+    """J = rate_misfit of the producer surface rates against data, and
+    (dJ/d lnk, dJ/d well values) by torch autograd; segments of checkpoint_every
+    steps keep only their input state and are recomputed (solves included)."""
+    from torch.utils.checkpoint import checkpoint
+
+    lnk = lnk.detach().requires_grad_(True)
+    values = values.detach().requires_grad_(True)
+
+    def segment(s0, k, p, sw, sg, lnk, values):
+        J = torch.zeros((), dtype=p.dtype, device=p.device)
+        for t in range(s0, s0 + k):
+            out = []
+            p, sw, sg, _ = step(case, lnk, values, p, sw, sg, solve, direct, None, out)
+            J = J + rate_misfit(out[0], data[t], sigma[t])
+        return p, sw, sg, J
+
+    p, sw, sg = case.initial_state()
+    J = torch.zeros((), dtype=p.dtype, device=p.device)
+    done = 0
+    while done < steps:
+        k = min(checkpoint_every or steps, steps - done)
+        if checkpoint_every:
+            p, sw, sg, Js = checkpoint(segment, done, k, p, sw, sg, lnk, values, use_reentrant=False)
+        else:
+            p, sw, sg, Js = segment(done, k, p, sw, sg, lnk, values)
+        J = J + Js
+        done += k
+    g_lnk, g_values = torch.autograd.grad(J, (lnk, values))
+    return J.detach(), g_lnk, g_values

@@ -599,6 +599,65 @@ int impes_model_set_error_tol(impes_model* m, double etol) {
   });
 }
 
+// ------------------------------------------------------------------ model adjoint
+int impes_model_adjoint_reset(impes_model* m, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m, "null pointer");
+    m->m.adjoint_reset(as_stream(stream));
+  });
+}
+
+int impes_model_transport_vjp(impes_model* m, double dt, const double* p_prev, const double* p_new,
+                              const double* sw, const double* sg, const double* g_sw_next, const double* g_sg_next,
+                              const double* g_obs_host, double* g_p_new, double* g_p_prev, double* g_sw, double* g_sg,
+                              void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p_prev && p_new && sw && sg && g_sw_next && g_sg_next && g_p_new && g_p_prev && g_sw && g_sg,
+                  "null pointer");
+    IMPES_REQUIRE(dt > 0.0, "dt must be positive");
+    IMPES_REQUIRE(g_obs_host || m->m.num_wells() == 0, "null observation cotangents");
+    m->m.transport_vjp(dt, p_prev, p_new, sw, sg, g_sw_next, g_sg_next, g_obs_host, g_p_new, g_p_prev, g_sw, g_sg,
+                       as_stream(stream));
+  });
+}
+
+int impes_model_solve_adjoint(impes_model* m, int precision, double rtol, int restart, int maxit, const double* g,
+                              double* lam, impes_solve_stats* stats, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && g && lam, "null pointer");
+    IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+    IMPES_REQUIRE(rtol > 0.0 && maxit >= 1, "bad tolerance / iteration limit");
+    SolveStats st;
+    m->m.solve_adjoint(precision, rtol, restart, maxit, g, lam, st, as_stream(stream));
+    fill_stats(stats, st);
+  });
+}
+
+int impes_model_assemble_vjp(impes_model* m, double dt, const double* p_prev, const double* sw, const double* sg,
+                             const double* p_new, const double* lam, double* g_p_prev, double* g_sw, double* g_sg,
+                             void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && p_prev && sw && sg && p_new && lam && g_p_prev && g_sw && g_sg, "null pointer");
+    IMPES_REQUIRE(dt > 0.0, "dt must be positive");
+    m->m.assemble_vjp(dt, p_prev, sw, sg, p_new, lam, g_p_prev, g_sw, g_sg, as_stream(stream));
+  });
+}
+
+int impes_model_rock_vjp(impes_model* m, const double* kx, const double* ky, const double* kz, double* g_kx,
+                         double* g_ky, double* g_kz, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && kx && ky && kz && g_kx && g_ky && g_kz, "null pointer");
+    m->m.rock_vjp(kx, ky, kz, g_kx, g_ky, g_kz, as_stream(stream));
+  });
+}
+
+int impes_model_adjoint_values(impes_model* m, double* g_values_host, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m && (g_values_host || m->m.num_wells() == 0), "null pointer");
+    m->m.adjoint_values(g_values_host, as_stream(stream));
+  });
+}
+
 int impes_model_solve_cg(impes_model* m, double tol, int maxit, double* p_new, int* converged, int* iterations,
                          void* stream) {
   return guarded(__func__, [&] {

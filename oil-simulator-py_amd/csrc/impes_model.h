@@ -30,7 +30,9 @@ enum WellSlot {
   kWQTotal = 4,
   kWSurf = 5,      // 4 surface rates: oil, water, gas, liquid (m3/day)
   kWBal = 9,       // row-balance rate of a BHP well (m3/s), NaN if the direct form is used
-  kWellSlots = 12,
+  kWObs = 12,      // producers: surface oil, water, gas rates (m3/day) of every rate and BHP
+                   // producer (formation volume factors 1 without a PVT table); 0 for injectors
+  kWellSlots = 16,
 };
 
 constexpr int kWellScratch = 9;  // per-well scratch row of the transport step's well loop
@@ -74,6 +76,31 @@ class Model {
   void eval_props(long long n, const double* p, const double* sw, const double* sg, double* out, hipStream_t s);
   void eval_pvt(int key, long long n, const double* p, double* out, hipStream_t s);
 
+  // ---- reverse sweep (adjoint.hip): see the C ABI for the contracts
+  // zero the accumulated transmissibility and well-control cotangents
+  void adjoint_reset(hipStream_t s);
+  // transport of the step assembled at (p_prev, sw, sg): g_sw0/g_sg0 = cotangents of
+  // (sw, sg); adds into g_p_new and g_p_prev; g_obs_host: nw x 3 cotangents of the
+  // producer surface rates
+  void transport_vjp(double dt, const double* p_prev, const double* p_new, const double* sw, const double* sg,
+                     const double* g_sw1, const double* g_sg1, const double* g_obs_host, double* g_p_new,
+                     double* g_p_prev, double* g_sw0, double* g_sg0, hipStream_t s);
+  // lambda = A^-1 g with the assembled (symmetric) operator and the hierarchy
+  void solve_adjoint(int precision, double rtol, int restart, int maxit, const double* g, double* lam,
+                     SolveStats& st, hipStream_t s);
+  // assembly of (p_prev, sw, sg) given lambda and the solved p_new: adds into
+  // g_p_prev, g_sw0, g_sg0 (after transport_vjp of the same step)
+  void assemble_vjp(double dt, const double* p_prev, const double* sw, const double* sg, const double* p_new,
+                    const double* lam, double* g_p_prev, double* g_sw0, double* g_sg0, hipStream_t s);
+  // accumulated transmissibility cotangents -> permeability cotangents (m^2)
+  void rock_vjp(const double* kx, const double* ky, const double* kz, double* g_kx, double* g_ky, double* g_kz,
+                hipStream_t s);
+  void adjoint_values(double* host, hipStream_t s);
+  // forward pieces the reverse recomputes
+  void transport_forward(double dt, const double* p_prev, const double* p_new, const double* sw, const double* sg,
+                         hipStream_t s);
+  void pressure_props(double dt, const double* p, const double* sw, const double* sg, hipStream_t s);
+
   long long n() const { return N_; }
   int num_wells() const { return nw_; }
   int report_len() const { return kReportHead + kWellSlots * nw_ + (n_env_ > 1 ? n_env_ : 0); }
@@ -113,6 +140,12 @@ class Model {
   std::unique_ptr<Hierarchy> amg_;
   Fgmres fgmres_;
   DevBuf<double> cg_r_, cg_z_, cg_p_, cg_ap_, cg_s_;
+  // adjoint workspace (allocated on first use)
+  void ensure_adjoint();
+  DevBuf<double> gtx_, gty_, gtz_, g_val_;  // accumulated over the reverse sweep
+  DevBuf<double> g_div_w_, g_div_g_, g_mw_, g_mo_, g_mg_, g_rw_, g_ro_, g_rg_, g_pc_, g_mt_p_, g_acc_, g_extra_;
+  DevBuf<double> g_obs_, g_ws_;
+  Fgmres adj_fgmres_;
   double dt_ = 0.0;
 };
 

@@ -1,5 +1,6 @@
 // IMPES step kernels and their host orchestration.
 #include "impes_model.h"
+#include "model_dev.h"
 
 #include <cstring>
 
@@ -13,26 +14,8 @@
 namespace impes {
 
 namespace {
-constexpr int kBlock = 256;
-constexpr double kDay = 86400.0;
+using namespace model_dev;
 constexpr int kLedgerQ = 10;  // mass(6) + volume dV_w, dV_g, clamp_w, clamp_g
-
-struct GridDev {
-  int nx, ny, nz;
-  double dx, dy, dz;
-  long long sx, sy;
-  int gravity;
-  int env_nx;  // > 0: independent reservoirs of env_nx x-layers each, stacked along x
-};
-
-#define CELL_LOOP(c, n) for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < (n); c += (long long)gridDim.x * blockDim.x)
-
-__device__ __forceinline__ void ijk(const GridDev& G, long long c, int& i, int& j, int& k) {
-  k = (int)(c % G.nz);
-  const long long t = c / G.nz;
-  j = (int)(t % G.ny);
-  i = (int)(t / G.ny);
-}
 
 // ---------------------------------------------------------------- transmissibility
 // simulation.py:52-74: T = 2 k1 k2/(k1 + k2 + 1e-15) * area / distance
@@ -331,8 +314,6 @@ __device__ double row_balance(const SatWellCtx& X, int c, double q_rate) {
   return f + X.acc[c] * (pc - X.p_prev[c]) - q_rate - X.extra[c];
 }
 
-__device__ __forceinline__ double mx12(double a) { return fmax(a, 1e-12); }
-
 // The per-well loop of the transport step (simulation.py:1051-1347). Each well's
 // terms depend only on its cell's state and on the pressure step's report, so one
 // thread per well computes them into its scratch row ws[9 w ..] (cell increments
@@ -356,6 +337,7 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
     double* R = rep + kReportHead + kWellSlots * w;
     R[kWHasQ] = 0.0;
     R[kWBal] = NAN;
+    R[kWObs + 0] = R[kWObs + 1] = R[kWObs + 2] = 0.0;
     const int c = d.cell;
     const double P = X.p[c];
     const PvtAt atP = pvt ? pvt_locate(X.T, P) : PvtAt{};  // every lookup below is at P
@@ -431,9 +413,16 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
           R[kWSurf + 1] = ws;
           R[kWSurf + 2] = gs;
           R[kWSurf + 3] = os + ws;
+          R[kWObs + 0] = os;
+          R[kWObs + 1] = ws;
+          R[kWObs + 2] = gs;
           const double qo_res = -q * fo, qg_res = -q * fg;
           oil_out += dt * (fmax(qo_res, 0.0) / mx12(bo) + rv * fmax(qg_res, 0.0) / mx12(bg));
           gas_out += dt * (fmax(qg_res, 0.0) / mx12(bg) + rs * fmax(qo_res, 0.0) / mx12(bo));
+        } else {
+          R[kWObs + 0] = fabs(q) * fo * kDay;
+          R[kWObs + 1] = fabs(q) * fw * kDay;
+          R[kWObs + 2] = fabs(q) * fg * kDay;
         }
         dqw += q * fw;
         dqg += q * fg;
@@ -480,13 +469,20 @@ __global__ __launch_bounds__(256) void k_wells_sat(SatWellCtx X, double* __restr
         dqo += -q * fo;
         sum_qw += -q * fw;
         sum_qg += -q * fg;
+        const double qa = fabs(q);
         if (pvt) {
           const double bo = pvt_at(X.T, kBo, atP), bg = pvt_at(X.T, kBg, atP);
-          const double rs = pvt_at(X.T, kRs, atP), rv = pvt_at(X.T, kRv, atP);
-          const double qa = fabs(q);
+          const double rs = pvt_at(X.T, kRs, atP), rv = pvt_at(X.T, kRv, atP), bw = pvt_at(X.T, kBw, atP);
           const double qo_res = qa * fo, qg_res = qa * fg;
           oil_out += dt * (qo_res / mx12(bo) + rv * qg_res / mx12(bg));
           gas_out += dt * (qg_res / mx12(bg) + rs * qo_res / mx12(bo));
+          R[kWObs + 0] = qa * (fo / mx12(bo) + fg * rv / mx12(bg)) * kDay;
+          R[kWObs + 1] = qa * (fw / mx12(bw)) * kDay;
+          R[kWObs + 2] = qa * (fg / mx12(bg) + fo * rs / mx12(bo)) * kDay;
+        } else {
+          R[kWObs + 0] = qa * fo * kDay;
+          R[kWObs + 1] = qa * fw * kDay;
+          R[kWObs + 2] = qa * fg * kDay;
         }
       }
     }
@@ -808,17 +804,6 @@ __global__ void k_eval_pvt(PvtDev T, int key, long long n, const double* __restr
   CELL_LOOP(c, n) out[c] = pvt_eval(T, key, p[c]);
 }
 
-GridDev grid_dev(const Grid& g, bool gravity, int env_nx) {
-  GridDev G;
-  G.nx = g.nx; G.ny = g.ny; G.nz = g.nz;
-  G.dx = g.dx; G.dy = g.dy; G.dz = g.dz;
-  G.sx = (long long)g.ny * g.nz;
-  G.sy = g.nz;
-  G.gravity = gravity ? 1 : 0;
-  G.env_nx = env_nx;
-  return G;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------- Model
@@ -1027,6 +1012,39 @@ void Model::saturation(double dt, const double* p_prev, const double* p_new, con
   }
   IMPES_HIP_CHECK(hipMemcpyAsync(host_report, report_.p, sizeof(double) * report_len(), hipMemcpyDeviceToHost, s));
   IMPES_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void Model::transport_forward(double dt, const double* p_prev, const double* p_new, const double* sw,
+                              const double* sg, hipStream_t s) {
+  GridDev G = grid_dev(grid, gravity_, env_nx_);
+  const int gr = grid_for(N_, kBlock, 1 << 16);
+  k_props_sat<<<gr, kBlock, 0, s>>>(G, F, pvt, p_new, sw, sg, mob_w_.p, mob_o_.p, mob_g_.p, rho_w_.p, rho_g_.p,
+                                    pcog_.p);
+  IMPES_LAUNCH_CHECK();
+  k_flux<<<gr, kBlock, 0, s>>>(G, F, p_new, tx.p, ty.p, tz.p, mob_w_.p, mob_g_.p, rho_w_.p, rho_g_.p, pcog_.p,
+                               div_w_.p, div_g_.p);
+  IMPES_LAUNCH_CHECK();
+  SatWellCtx X{G, F, pvt, dt, nw_, wells_.p, p_prev, p_new, sw, sg, pv.p, mob_w_.p, mob_o_.p, mob_g_.p, mob_t.p,
+               ox.p, oy.p, oz.p, acc.p, extra.p};
+  k_wells_sat<<<1, 256, 0, s>>>(X, qw_.p, qg_.p, qo_.p, report_.p, well_sc_.p);
+  IMPES_LAUNCH_CHECK();
+  k_rat<<<kRedBlocks, kBlock, 0, s>>>(G, F, dt, sw, sg, pv.p, qw_.p, qg_.p, div_w_.p, div_g_.p, dsw_.p, dsg_.p,
+                                      part_.p, nullptr);
+  IMPES_LAUNCH_CHECK();
+  k_nsub<<<1, kBlock, 0, s>>>(kRedBlocks, F, part_.p, report_.p);
+  IMPES_LAUNCH_CHECK();
+  if (nw_ > 0) {
+    k_reset_wells<<<1, 1, 0, s>>>(nw_, wells_.p, qw_.p, qg_.p, qo_.p);
+    IMPES_LAUNCH_CHECK();
+  }
+}
+
+void Model::pressure_props(double dt, const double* p, const double* sw, const double* sg, hipStream_t s) {
+  GridDev G = grid_dev(grid, gravity_, env_nx_);
+  k_props_pressure<<<grid_for(N_, kBlock, 1 << 16), kBlock, 0, s>>>(G, F, pvt, dt, p, sw, sg, pv.p, mob_w_.p,
+                                                                    mob_o_.p, mob_g_.p, mob_t.p, rho_w_.p, rho_o_.p,
+                                                                    rho_g_.p, pc_.p, acc.p);
+  IMPES_LAUNCH_CHECK();
 }
 
 void Model::eval_props(long long n, const double* p, const double* sw, const double* sg, double* out, hipStream_t s) {

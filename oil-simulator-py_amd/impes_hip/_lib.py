@@ -150,6 +150,12 @@ SIGNATURES = {
     "impes_model_solve_cg": [_P, _D, _I, _P, _PI, _PI, _P],
     "impes_model_saturation": [_P, _D, _P, _P, _P, _P, _P, _P, _P, _PD, _P],
     "impes_model_export": [_P, _I, _P, _P],
+    "impes_model_adjoint_reset": [_P, _P],
+    "impes_model_transport_vjp": [_P, _D, _P, _P, _P, _P, _P, _P, _PD, _P, _P, _P, _P, _P],
+    "impes_model_solve_adjoint": [_P, _I, _D, _I, _I, _P, _P, C.POINTER(SolveStats), _P],
+    "impes_model_assemble_vjp": [_P, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "impes_model_rock_vjp": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "impes_model_adjoint_values": [_P, _PD, _P],
     "impes_model_eval_props": [_P, _L, _P, _P, _P, _P, _P],
     "impes_model_eval_pvt": [_P, _I, _L, _P, _P, _P],
     "impes_model_amg_levels": [_P, _PI, _PI, _PI, _I],

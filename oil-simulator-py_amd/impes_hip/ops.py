@@ -288,7 +288,7 @@ class FgmresSolver:
 
 # ---------------------------------------------------------------------- IMPES model
 REPORT_HEAD = 32
-WELL_SLOTS = 12
+WELL_SLOTS = 16
 EXPORT_FIELDS = {"diag": 0, "ox": 1, "oy": 2, "oz": 3, "rhs": 4, "acc": 5, "extra": 6, "tx": 7, "ty": 8,
                  "tz": 9, "pv": 10, "mob_t": 11, "dinv": 12}
 PVT_KEYS = {"Bo": 0, "Bw": 1, "Bg": 2, "mu_o": 3, "mu_w": 4, "mu_g": 5, "Rs": 6, "Rv": 7}
@@ -373,6 +373,42 @@ class ImpesModel:
         call("impes_model_solve_pressure", self._h, int(precision), float(rtol), int(restart), int(maxit),
              ptr(p_prev), ptr(p_new), C.byref(st), stream_ptr())
         return st.as_dict()
+
+    # ---- reverse sweep (hand-written adjoint of the step)
+    def adjoint_reset(self):
+        call("impes_model_adjoint_reset", self._h, stream_ptr())
+
+    def transport_vjp(self, dt, p_prev, p_new, sw, sg, g_sw_next, g_sg_next, g_obs, g_p_new, g_p_prev, g_sw, g_sg):
+        """g_obs: host array (nwells, 3). Writes g_sw, g_sg; adds into g_p_new, g_p_prev."""
+        import numpy as np
+
+        go = np.ascontiguousarray(np.asarray(g_obs, dtype=np.float64).reshape(-1))
+        buf = (C.c_double * max(go.size, 1))(*go.tolist())
+        call("impes_model_transport_vjp", self._h, float(dt), ptr(p_prev), ptr(p_new), ptr(sw), ptr(sg),
+             ptr(g_sw_next), ptr(g_sg_next), buf, ptr(g_p_new), ptr(g_p_prev), ptr(g_sw), ptr(g_sg), stream_ptr())
+
+    def solve_adjoint(self, g, lam, *, precision=32, rtol=1e-10, restart=30, maxit=400) -> dict:
+        st = SolveStats()
+        call("impes_model_solve_adjoint", self._h, int(precision), float(rtol), int(restart), int(maxit), ptr(g),
+             ptr(lam), C.byref(st), stream_ptr())
+        return st.as_dict()
+
+    def assemble_vjp(self, dt, p_prev, sw, sg, p_new, lam, g_p_prev, g_sw, g_sg):
+        call("impes_model_assemble_vjp", self._h, float(dt), ptr(p_prev), ptr(sw), ptr(sg), ptr(p_new), ptr(lam),
+             ptr(g_p_prev), ptr(g_sw), ptr(g_sg), stream_ptr())
+
+    def rock_vjp(self, kx, ky, kz):
+        g = [torch.empty_like(kx) for _ in range(3)]
+        call("impes_model_rock_vjp", self._h, ptr(kx), ptr(ky), ptr(kz), ptr(g[0]), ptr(g[1]), ptr(g[2]),
+             stream_ptr())
+        return g
+
+    def adjoint_values(self):
+        import numpy as np
+
+        buf = (C.c_double * max(self.nwells, 1))()
+        call("impes_model_adjoint_values", self._h, buf, stream_ptr())
+        return np.array(buf[:self.nwells], dtype=np.float64)
 
     def solve_cg(self, p_new, *, tol, maxit) -> tuple[bool, int]:
         conv, its = C.c_int(), C.c_int()

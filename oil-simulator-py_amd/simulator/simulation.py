@@ -28,7 +28,7 @@ import torch
 from .well import LIMIT_KEYS, SURFACE_PHASES
 
 REPORT_HEAD = 32
-WELL_SLOTS = 12
+WELL_SLOTS = 16
 
 
 def _flat(t: torch.Tensor) -> torch.Tensor:
