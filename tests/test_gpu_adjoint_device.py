@@ -23,7 +23,11 @@ producer_limits amplifies the forward's pressure-solve roundoff ~30-fold per
 step (device and restatement observations differ by 2e-14, 2e-13, 7e-12, 2e-10
 relative over the four steps: growth, not a formula difference, which would
 show at the first step), so its two runs are compared at 1e-8 (observations)
-and 1e-6 (gradients); the finite-difference check holds it to the same 1e-4."""
+and 1e-6 (gradients); the finite-difference check holds it to the same 1e-4, at
+h = 1e-5: kinks the replay does not fix (relative-permeability clamps at the
+saturation bounds, PVT table nodes) lie within 1e-4 of the prior in ln k
+(replayed differences along its directions: 0.6 / 3e-2 / 1e-5 relative error at
+h = 1e-3 / 1e-4 / 1e-5)."""
 import json
 import os
 
@@ -42,6 +46,7 @@ STEPS = 4
 TIGHT = {"rtol": 1e-13, "adj_rtol": 1e-13, "etol": 0.0, "maxit": 600}
 # (observations, gradients) relative bars against the restatement
 BARS = {"producer_limits": (1e-8, 1e-6)}
+FD_H = {"producer_limits": 1e-5}
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -132,7 +137,7 @@ def test_device_gradient_matches_replayed_finite_differences(problem):
     J0 = torch_misfit(P, P["prior"], base)
     assert abs(J0 - J) <= BARS.get(P["name"], (1e-10, 1e-9))[0] * abs(J)
     rng = np.random.default_rng(SEEDS["fd"])
-    h = 1e-4
+    h = FD_H.get(P["name"], 1e-4)
     worst = 0.0
     for _ in range(20):
         v = torch.tensor(rng.standard_normal(tuple(P["prior"].shape)))
