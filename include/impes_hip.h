@@ -78,7 +78,8 @@ typedef struct {
   int post_smooth;       /* V-cycle post-sweeps (2) */
   double omega;          /* L1-Jacobi damping (default 0.8; reference 0.7) */
   double equil_threshold;/* equilibrate when the median |a_ii| < threshold (1e-6) */
-  int dense_max;         /* coarsest level solved densely up to this size (500) */
+  int dense_max;         /* coarsest level solved densely up to this size (500; at most 4096: above
+                          * 512 rows the inverse is formed on the matrix cores, MFMA FP64) */
   int coarse_sweeps;     /* Jacobi sweeps on a larger coarsest level (50) */
   int anchor;            /* forced C point on every level, -1 = none (reference anchor_idx) */
   /* Preconditioner choices that are not in the reference; they change the iteration

@@ -221,6 +221,9 @@ struct Sel<double> {
   static void rj0(const CsrF64& R, const double* r, const double* ir, double w, double* bc, double* xc, int g,
                   hipStream_t s) { launch_restrict_j0_f64(R, r, ir, w, bc, xc, g, s); }
   static void pin0(const int* pin, double* b, hipStream_t s) { launch_pin_rhs_f64(pin, b, s); }
+  static void dense_mv(int n, const int* pin, const double* M, const double* b, double* x, hipStream_t s) {
+    launch_dense_mv_f64(n, pin, M, b, x, s);
+  }
   static void dense(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s) {
     launch_dense_solve_f64(n, pin, aug, b, x, s);
   }
@@ -254,6 +257,9 @@ struct Sel<float> {
   static void rj0(const CsrF32& R, const float* r, const float* ir, double w, float* bc, float* xc, int g,
                   hipStream_t s) { launch_restrict_j0_f32(R, r, ir, (float)w, bc, xc, g, s); }
   static void pin0(const int* pin, float* b, hipStream_t s) { launch_pin_rhs_f32(pin, b, s); }
+  static void dense_mv(int n, const int* pin, const double* M, const float* b, float* x, hipStream_t s) {
+    launch_dense_mv_f32(n, pin, M, b, x, s);
+  }
   static void dense(int n, const int* pin, const double* aug, const float* b, float* x, hipStream_t s) {
     launch_dense_solve_f32(n, pin, aug, b, x, s);
   }
@@ -615,9 +621,26 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
   L.pin.alloc(2);  // [0] pin flag, [1] inverse produced by the SPD sweep
   launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
   if (!L.dense) return;
+  L.mfma = L.n > 512;
+  if (L.mfma) {
+    const long long np = dense_pad(L.n);
+    L.inv.alloc(np * np);
+    L.gj_f.alloc(np * 64);
+    L.gj_p.alloc(64 * 64);
+    L.gj_bad.alloc(1);
+    dense_inverse(L, s);
+    IMPES_REQUIRE(read_scalar(L.gj_bad.p, s) == 0,
+                  "amg setup: coarsest operator is not symmetric positive definite (lower dense_max)");
+    return;
+  }
   L.aug.alloc(2LL * L.n * L.n);
   launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
   launch_gauss_jordan_inverse(L.n, L.aug.p, L.pin.p + 1, s);
+}
+
+void Hierarchy::dense_inverse(Level& L, hipStream_t s) {
+  launch_fill_i32(L.gj_bad.p, 0, 1, s);
+  launch_dense_inverse_mfma(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.inv.p, L.gj_f.p, L.gj_p.p, L.gj_bad.p, s);
 }
 
 void Hierarchy::refresh_f32(hipStream_t s) {
@@ -674,7 +697,9 @@ void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
     }
     if (L.coarsest) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
-      if (L.dense) {
+      if (L.dense && L.mfma) {
+        dense_inverse(L, s);
+      } else if (L.dense) {
         launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);
         launch_gauss_jordan_inverse(L.n, L.aug.p, L.pin.p + 1, s);
       }
@@ -744,6 +769,10 @@ void Hierarchy::smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& z
 template <class T>
 void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
   Level& L = levels_[l];
+  if (L.dense && L.mfma) {
+    Sel<T>::dense_mv(L.n, L.pin.p, L.inv.p, b, x, s);
+    return;
+  }
   if (L.dense) {
     Sel<T>::dense(L.n, L.pin.p, L.aug.p, b, x, s);
     return;

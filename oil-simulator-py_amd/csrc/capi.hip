@@ -86,7 +86,7 @@ static AmgParams params_from_c(const impes_amg_params* c) {
   }
   IMPES_REQUIRE(p.max_levels >= 1 && p.max_levels <= 64, "max_levels out of range");
   IMPES_REQUIRE(p.pre_smooth >= 0 && p.post_smooth >= 0, "negative sweep count");
-  IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= 512, "dense_max must be in [1, 512]");
+  IMPES_REQUIRE(p.dense_max >= 1 && p.dense_max <= kDenseMfmaMax, "dense_max must be in [1, 4096]");
   IMPES_REQUIRE(p.equil_mode == kEquilSymmetric || p.equil_mode == kEquilScalar, "equil_mode must be 0 or 1");
   IMPES_REQUIRE(p.coarse_pin >= kPinNever && p.coarse_pin <= kPinAuto, "coarse_pin must be 0, 1 or 2");
   IMPES_REQUIRE(p.trunc >= 0.0 && p.trunc < 1.0, "trunc must be in [0, 1)");

@@ -1,0 +1,238 @@
+// Dense inverse of a large coarsest AMG operator (classical_amg.py:1045-1069 solves
+// the coarsest level densely) on the matrix cores: blocked Gauss-Jordan without
+// pivoting, whose rank-64 updates run as 64 x 64 x 64 FP64 tiles of
+// v_mfma_f64_16x16x4_f64. The operator is symmetric positive definite (a Galerkin
+// product of the SPD pressure matrix, with DOF 0 optionally pinned), so every
+// diagonal block met during the elimination is a Schur complement of an SPD
+// matrix and itself SPD: no pivoting is needed.
+//
+// The matrix (n x n, row-major) is padded to n_pad = 64 nb with an identity block.
+// For each block column k (one step of Gauss-Jordan on 64 columns at once):
+//   P = A_kk^-1                          (one workgroup, in LDS)
+//   A_kj <- P A_kj        (j != k)       row panel, MFMA
+//   F    <- A_ik          (i != k)       copy of the old column panel
+//   A_ij <- A_ij - F_i A_kj (i, j != k)  trailing update, MFMA
+//   A_ik <- -F_i P        (i != k)       MFMA
+//   A_kk <- P
+// which is the scalar in-place elimination (a_kj /= a_kk; a_ij -= a_ik a_kj;
+// a_ik = -a_ik / a_kk; a_kk = 1 / a_kk) on 64 x 64 blocks; after the last block A
+// holds A^-1. Each visit of the V-cycle then applies the inverse as one dense
+// matrix-vector product (a wavefront per row, FP64 accumulation).
+#include "amg_kernels.h"
+
+namespace impes {
+
+namespace {
+
+constexpr int kB = 64;        // block size of the elimination
+constexpr int kLd = kB + 1;   // LDS row stride (doubles): rows fall on distinct banks
+constexpr int kGemmThreads = 256;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// C = A B for 64 x 64 tiles in LDS (row stride kLd). Wave w owns rows 16w..16w+15
+// as four 16 x 16 output tiles t (columns 16t..16t+15). v_mfma_f64_16x16x4_f64 lane
+// layout (measured on gfx950): A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
+// D[i][j] at lane j + 16 (i % 4), element i / 4.
+__device__ __forceinline__ void tile_mma(const double* __restrict__ As, const double* __restrict__ Bs, v4d acc[4]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k0 = 0; k0 < kB; k0 += 4) {
+    const double a = As[(16 * w + i) * kLd + k0 + kk];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const double b = Bs[(k0 + kk) * kLd + 16 * t + i];
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+  }
+}
+
+// (row, column) within the 64 x 64 tile of element e of output tile t
+__device__ __forceinline__ void tile_pos(int t, int e, int& r, int& c) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  r = 16 * w + 4 * e + (lane >> 4);
+  c = 16 * t + (lane & 15);
+}
+
+// global 64 x 64 block (row-major, leading dimension ld) -> LDS tile
+__device__ __forceinline__ void load_tile(const double* __restrict__ g, long long ld, double* __restrict__ s) {
+  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
+    const int r = q / kB, c = q % kB;
+    s[r * kLd + c] = g[r * ld + c];
+  }
+}
+
+// Dense n_pad x n_pad from CSR: one workgroup per row; the row is staged in LDS,
+// its entries added in entry order by one thread (repeated columns add up as in a
+// serial scatter), then written out. Padding rows are identity rows; a pinned DOF
+// 0 keeps only a_00 = 1 in its row and column.
+__global__ __launch_bounds__(256) void k_fill_padded(int n, int n_pad, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const double* __restrict__ v,
+                                                     const int* __restrict__ pin_flag, double* __restrict__ M) {
+  extern __shared__ double row[];
+  const int r = blockIdx.x;
+  const int pin = *pin_flag;
+  for (int c = threadIdx.x; c < n_pad; c += blockDim.x) row[c] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (r >= n || (pin && r == 0)) {
+      row[r] = 1.0;
+    } else {
+      for (int e = rp[r]; e < rp[r + 1]; ++e) {
+        const int c = ci[e];
+        if (pin && c == 0) continue;
+        row[c] += v[e];
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < n_pad; c += blockDim.x) M[(long long)r * n_pad + c] = row[c];
+}
+
+// P = A_kk^-1 by unpivoted Gauss-Jordan in LDS (one workgroup). A non-positive
+// pivot (not SPD) sets *bad.
+__global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double* __restrict__ M,
+                                                 double* __restrict__ P, int* __restrict__ bad) {
+  __shared__ double a[kB * kLd];
+  __shared__ double col[kB];
+  const long long off = (long long)k * kB * n_pad + (long long)k * kB;
+  load_tile(M + off, n_pad, a);
+  __syncthreads();
+  for (int p = 0; p < kB; ++p) {
+    const double piv = a[p * kLd + p];
+    if (threadIdx.x == 0 && !(piv > 0.0)) *bad = 1;
+    // stage column p (old), then scale row p with a_pp = 1 replaced first
+    for (int i = threadIdx.x; i < kB; i += blockDim.x) col[i] = a[i * kLd + p];
+    __syncthreads();
+    const double inv = 1.0 / piv;
+    for (int j = threadIdx.x; j < kB; j += blockDim.x) a[p * kLd + j] = ((j == p) ? 1.0 : a[p * kLd + j]) * inv;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
+      const int i = q / kB, j = q % kB;
+      if (i == p) continue;
+      const double f = col[i];
+      a[i * kLd + j] = ((j == p) ? 0.0 : a[i * kLd + j]) - f * a[p * kLd + j];
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) P[q] = a[(q / kB) * kLd + q % kB];
+}
+
+// Row panel: A_kj <- P A_kj for every column block j != k (one workgroup each).
+__global__ __launch_bounds__(kGemmThreads) void k_gj_row(int k, int n_pad, double* __restrict__ M,
+                                                         const double* __restrict__ P) {
+  __shared__ double As[kB * kLd], Bs[kB * kLd];
+  const int j = blockIdx.x;
+  if (j == k) return;
+  double* blk = M + (long long)k * kB * n_pad + (long long)j * kB;
+  load_tile(P, kB, As);
+  load_tile(blk, n_pad, Bs);
+  __syncthreads();
+  v4d acc[4];
+  tile_mma(As, Bs, acc);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int r, c;
+      tile_pos(t, e, r, c);
+      blk[(long long)r * n_pad + c] = acc[t][e];
+    }
+}
+
+// F = column panel k (n_pad x 64), before the trailing update overwrites it.
+__global__ void k_gj_col(int k, int n_pad, const double* __restrict__ M, double* __restrict__ F) {
+  const long long total = (long long)n_pad * kB;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total; q += (long long)gridDim.x * blockDim.x) {
+    const long long r = q / kB;
+    const int c = (int)(q % kB);
+    F[q] = M[r * n_pad + (long long)k * kB + c];
+  }
+}
+
+// Trailing update of block (i, j), i != k: A_ij -= F_i A_kj (j != k) or
+// A_ik = -F_i P (j == k). One workgroup per block; blockIdx.x = j, blockIdx.y = i.
+__global__ __launch_bounds__(kGemmThreads) void k_gj_update(int k, int n_pad, double* __restrict__ M,
+                                                            const double* __restrict__ F,
+                                                            const double* __restrict__ P) {
+  __shared__ double As[kB * kLd], Bs[kB * kLd];
+  const int j = blockIdx.x, i = blockIdx.y;
+  if (i == k) return;
+  load_tile(F + (long long)i * kB * kB, kB, As);
+  if (j == k) load_tile(P, kB, Bs);
+  else load_tile(M + (long long)k * kB * n_pad + (long long)j * kB, n_pad, Bs);
+  __syncthreads();
+  v4d acc[4];
+  tile_mma(As, Bs, acc);
+  double* blk = M + (long long)i * kB * n_pad + (long long)j * kB;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int r, c;
+      tile_pos(t, e, r, c);
+      double* dst = blk + (long long)r * n_pad + c;
+      *dst = (j == k) ? -acc[t][e] : *dst - acc[t][e];
+    }
+}
+
+__global__ void k_gj_put_diag(int k, int n_pad, double* __restrict__ M, const double* __restrict__ P) {
+  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x)
+    M[(long long)(k * kB + q / kB) * n_pad + (long long)k * kB + q % kB] = P[q];
+}
+
+// x = A^-1 b (b_0 taken as 0 when DOF 0 is pinned): a wavefront per row.
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_mv(int n, int n_pad, const int* __restrict__ pin_flag,
+                                                  const double* __restrict__ M, const T* __restrict__ b,
+                                                  T* __restrict__ x) {
+  const int pin = *pin_flag;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int nwaves = (gridDim.x * blockDim.x) / kWave;
+  for (int r = wid; r < n; r += nwaves) {
+    const double* row = M + (long long)r * n_pad;
+    double s = 0.0;
+    for (int c = (pin ? 1 : 0) + lane; c < n; c += kWave) s += row[c] * (double)b[c];
+    s = wave_sum(s);
+    if (lane == 0) x[r] = (T)s;
+  }
+}
+
+}  // namespace
+
+int dense_pad(int n) { return div_up(n, kB) * kB; }
+
+void launch_dense_inverse_mfma(int n, const int* rp, const int* ci, const double* v, const int* pin, double* M,
+                               double* F, double* P, int* bad, hipStream_t s) {
+  IMPES_REQUIRE(n >= 1 && n <= kDenseMfmaMax, "MFMA dense inverse supports 1 <= n <= 4096");
+  const int n_pad = dense_pad(n), nb = n_pad / kB;
+  k_fill_padded<<<n_pad, 256, n_pad * sizeof(double), s>>>(n, n_pad, rp, ci, v, pin, M);
+  IMPES_LAUNCH_CHECK();
+  for (int k = 0; k < nb; ++k) {
+    k_gj_diag<<<1, 256, 0, s>>>(k, n_pad, M, P, bad);
+    IMPES_LAUNCH_CHECK();
+    k_gj_row<<<nb, kGemmThreads, 0, s>>>(k, n_pad, M, P);
+    IMPES_LAUNCH_CHECK();
+    k_gj_col<<<div_up(n_pad * kB, 256), 256, 0, s>>>(k, n_pad, M, F);
+    IMPES_LAUNCH_CHECK();
+    k_gj_update<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, M, F, P);
+    IMPES_LAUNCH_CHECK();
+    k_gj_put_diag<<<1, 256, 0, s>>>(k, n_pad, M, P);
+    IMPES_LAUNCH_CHECK();
+  }
+}
+
+void launch_dense_mv_f64(int n, const int* pin, const double* M, const double* b, double* x, hipStream_t s) {
+  k_dense_mv<double><<<div_up(n, 4), 256, 0, s>>>(n, dense_pad(n), pin, M, b, x);
+  IMPES_LAUNCH_CHECK();
+}
+
+void launch_dense_mv_f32(int n, const int* pin, const double* M, const float* b, float* x, hipStream_t s) {
+  k_dense_mv<float><<<div_up(n, 4), 256, 0, s>>>(n, dense_pad(n), pin, M, b, x);
+  IMPES_LAUNCH_CHECK();
+}
+
+}  // namespace impes
