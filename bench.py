@@ -35,6 +35,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-dims", default="50x50x40")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="concurrent single-threaded oracle copies of the CPU baseline (0: one per available CPU, "
+                         "at most 16)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cold-steps", type=int, default=20, help="steps of the cold run (setup included)")
     ap.add_argument("--kernels", default="csr_jacobi_f32,st7_jacobi_f32",
                     help="kernels timed for the roofline figures: the first (the dominant one, the L1-Jacobi "
                          "sweep of the CSR levels) is `roofline`, the others `roofline_other`")
@@ -104,13 +109,9 @@ def run(dims, precision, steps, warmup, kernel_steps=0, kernels=()):
     return sim, el, its, kts, setup
 
 
-def cpu_baseline(dims, steps):
-    """The oracle (CPU restatement, same algorithm as the device path: FP64 AMG +
-    FGMRES to 1e-8) on a bounded sample of the workload: a dims-sized grid of the
-    same case (same field generator, wells placed proportionally), one untimed
-    step that builds the hierarchy, then `steps` timed steps."""
-    import numpy as np
-
+def _cpu_sample(dims, steps):
+    """One oracle run of the sample (see cpu_baseline); returns (timed seconds,
+    seconds of the hierarchy-building first step, the last pressure system)."""
     sys.path.insert(0, ROOT)
     from oracle import amg as oa, impes as oi
     from simulator.cases import baseline_case
@@ -125,19 +126,126 @@ def cpu_baseline(dims, steps):
             state["H"] = oa.Hierarchy(A, max_levels=25)
         else:
             state["H"].update(A)
+        state["A"], state["b"] = A, b
         x, st = oa.fgmres(A, b, x0=None, precond=state["H"].precondition, rtol=1e-8, maxit=400)
         return x
 
     dt = cfg["simulation"]["time_step_days"] * 86400.0
+    t0 = time.perf_counter()
     sim.step(dt, solver=solver)
+    t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     for _ in range(steps):
         sim.step(dt, solver=solver)
-    el = time.perf_counter() - t0
+    return time.perf_counter() - t0, t_first, state["A"], state["b"]
+
+
+def _cpu_worker(dims, steps):
+    """Child process of cpu_baseline: one single-threaded oracle run, its timed
+    seconds on stdout."""
+    el, _, _, _ = _cpu_sample(dims, steps)
+    print(json.dumps({"timed_s": el}), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(dims, steps, procs, splu_dims=(30, 30, 20)):
+    """The oracle (CPU restatement, same algorithm as the device path: FP64 AMG +
+    FGMRES to 1e-8) on a bounded sample of the workload: a dims-sized grid of the
+    same case (same field generator, wells placed proportionally), one untimed
+    step that builds the hierarchy, then `steps` timed steps. The oracle is
+    single-threaded (numpy/scipy sparse kernels), so the host is measured as
+    `procs` independent copies of the sample run at once in child processes, one
+    thread each (BLAS threads pinned to 1): value = procs x cells x steps / the
+    slowest copy's timed seconds. Beside it: one copy alone, and scipy's sparse
+    direct solve (SuperLU) of the sample's last pressure system against the
+    oracle's FGMRES-AMG solve of the same system, on a splu_dims grid of the same
+    case (SuperLU's fill on the 3D sample itself takes minutes)."""
+    import subprocess
+
+    import numpy as np
+    import scipy.sparse.linalg as spl
+
+    sys.path.insert(0, ROOT)
+    from oracle import amg as oa
+
     n = int(np.prod(dims))
-    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy/scipy, 1 thread) on a {dims[0]}x{dims[1]}x{dims[2]} grid of the same case, "
-                      f"{steps} steps after one hierarchy-building step, {el:.1f} s"}
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    args = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "x".join(map(str, dims)),
+            "--cpu-steps", str(steps)]
+    ps = [subprocess.Popen(args, env=env, stdout=subprocess.PIPE, text=True) for _ in range(procs)]
+    times = []
+    for p in ps:
+        out, _ = p.communicate(timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU baseline worker failed with exit code {p.returncode}")
+        times.append(json.loads(out.strip().splitlines()[-1])["timed_s"])
+    # one copy alone (this process, BLAS threads as set for it), then the direct solve
+    el1, t_first, _, _ = _cpu_sample(dims, steps)
+    _, _, A, b = _cpu_sample(splu_dims, 1)
+    t0 = time.perf_counter()
+    lu = spl.splu(A.tocsc())
+    x = lu.solve(b)
+    t_lu = time.perf_counter() - t0
+    H = oa.Hierarchy(A, max_levels=25)
+    t0 = time.perf_counter()
+    H.update(A)
+    xa, st = oa.fgmres(A, b, x0=None, precond=H.precondition, rtol=1e-8, maxit=400)
+    t_amg = time.perf_counter() - t0
+    return {"value": procs * n * steps / max(times), "unit": "cell-steps/s", "cores": procs, "kind": "port",
+            "sample": f"oracle (numpy/scipy, 1 thread per process) on a {dims[0]}x{dims[1]}x{dims[2]} grid of the "
+                      f"same case, {steps} steps after one hierarchy-building step, {procs} copies at once "
+                      f"(slowest {max(times):.1f} s)",
+            "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "single_process_value": n * steps / el1, "single_process_first_step_s": t_first,
+            "splu_dims": list(splu_dims), "splu_solve_s": t_lu, "fgmres_amg_solve_s": t_amg,
+            "splu_vs_fgmres_rel_diff": float(np.linalg.norm(x - xa) / np.linalg.norm(x))}
+
+
+def cold_run(dims, steps):
+    """A fresh simulator of the benchmark case run for `steps` steps from its
+    initial state, timed end to end: the first step builds the AMG hierarchy and
+    captures the V-cycle graphs, so the rate includes the setup."""
+    import torch
+
+    cfg, sim = make_sim(dims, "fp32")
+    dt = cfg["simulation"]["time_step_days"] * 86400.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not sim.run_step(dt):
+            raise RuntimeError("cold-run step failed")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    n = 1
+    for d in dims:
+        n *= d
+    return {"steps": steps, "total_s": el, "cell_steps_per_s_incl_setup": n * steps / el,
+            "hierarchy_setup_ms": 1e3 * sim._model.amg_setup_seconds()}
+
+
+def vcycle_level_table(sim, cycles=20):
+    """Per-level figures of the FP32 V-cycle of the benchmark's hierarchy:
+    each level's own launches timed with HIP events over `cycles` eager cycles,
+    and their algorithmic bytes per visit (row pointers, indices and values of
+    every operator pass once, each vector pass once)."""
+    ms, by, graph_ms = sim._model.amg_level_times(32, cycles)
+    rows = []
+    for l, ((n, nnz), t, b) in enumerate(zip(sim.amg_levels(), ms, by)):
+        rows.append({"level": l, "rows": n, "nnz": nnz, "us_per_visit": 1e3 * t, "bytes_per_visit": b,
+                     "GBs": b / (t * 1e-3) / 1e9 if t > 0 else None})
+    return {"levels": rows, "eager_cycle_us": 1e3 * sum(ms), "graph_cycle_us": 1e3 * graph_ms,
+            "bytes_per_cycle": sum(by)}
 
 
 def adjoint_timing(steps=None):
@@ -316,6 +424,9 @@ def rl_timing(num_envs, steps=8):
 
 def main():
     a = parse()
+    if a.cpu_worker:
+        _cpu_worker(tuple(int(x) for x in a.cpu_worker.split("x")), a.cpu_steps)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(multi-GPU sharding is described in the design notes, not implemented)", file=sys.stderr)
@@ -359,7 +470,10 @@ def main():
     names = [k for k in a.kernels.split(",") if k]
     sim_k, _, _, kts, _ = run(dims, "fp32", 0, a.warmup, kernel_steps=3, kernels=names)
     levels = sim_k.amg_levels()
+    extra["vcycle_levels"] = vcycle_level_table(sim_k)
     del sim_k
+    if a.cold_steps > 0:
+        extra["cold_run"] = cold_run(dims, a.cold_steps)
     tf = os.path.join(ROOT, "profiles", "hbm_traffic_1M_gas_injection.json")
     traffic_table = json.load(open(tf)) if os.path.exists(tf) else {}
 
@@ -390,7 +504,8 @@ def main():
     cpu = None
     if not a.no_cpu:
         cdims = tuple(int(x) for x in a.cpu_dims.split("x"))
-        cpu = cpu_baseline(cdims, a.cpu_steps)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        cpu = cpu_baseline(cdims, a.cpu_steps, a.cpu_procs or max(1, min(16, avail)))
 
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": 1e3 * el32 / a.steps, "higher_is_better": True, "scaling": "weak",

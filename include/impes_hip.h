@@ -114,6 +114,9 @@ int impes_amg_num_levels(const impes_amg* amg, int* out);
 int impes_amg_level_info(const impes_amg* amg, int level, int* n, int* nnz, int* p_nnz, int* is_coarsest,
                          int* dense);
 int impes_amg_equilibrated(const impes_amg* amg, int* out);
+/* The per-level V-cycle times of impes_model_amg_level_times for a hierarchy handle. */
+int impes_amg_level_times(impes_amg* amg, int precision, int cycles, double* ms_host, double* bytes_host,
+                          int max_levels, double* cycle_ms, void* stream);
 int impes_amg_setup_seconds(const impes_amg* amg, double* out);
 /* Copies level data into caller device buffers (sizes from level_info; any pointer may be NULL). */
 int impes_amg_export_level(const impes_amg* amg, int level, int* rowptr, int* col, double* val, int* p_rowptr,
@@ -335,6 +338,15 @@ int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_hos
  * solve): the setup part of ClassicalAMG.__init__ (classical_amg.py:699-938), which
  * the first pressure solve of a run performs and later solves update numerically. */
 int impes_model_amg_setup_seconds(const impes_model* m, double* out);
+/* Measurement, no counterpart in the reference: `cycles` eager V-cycles of the
+ * pressure solver's hierarchy on its current V-cycle input, with HIP events between
+ * the phases of every level. ms_host[l] (l < min(levels, max_levels)) = mean stream
+ * time per cycle of level l's own launches (sweeps, residual, restriction,
+ * prolongation, or the coarsest solve; children excluded); bytes_host[l] (may be
+ * NULL) = the algorithmic HBM bytes of those launches per visit; *cycle_ms = mean
+ * time of the same cycle replayed from its HIP graph. precision 32 or 64. */
+int impes_model_amg_level_times(impes_model* m, int precision, int cycles, double* ms_host, double* bytes_host,
+                                int max_levels, double* cycle_ms, void* stream);
 
 /* ------------------------------------------------------------------ measurement */
 /* No counterpart in the reference (instrumentation). Times every launch of one

@@ -9,6 +9,7 @@
 // preconditioner of the FP64 outer Krylov solver).
 #pragma once
 
+#include <array>
 #include <vector>
 
 #include "kernels.h"
@@ -73,6 +74,11 @@ struct Level {
   DevBuf<float> v32;
   DevBuf<double> ir64;
   DevBuf<float> ir32;
+  // FP32 operator padded to 4-entry rows (the FP32 cycle's sweeps and residuals)
+  bool pad4 = false;
+  int g4 = 1;
+  DevBuf<int> rp4, ci4;
+  DevBuf<float> v4;
   DevBuf<signed char> cf;  // final C/F markers (non-coarsest levels)
   // interpolation / restriction to the next level
   int nc = 0, pnnz = 0, pgroup = 1, rgroup = 1;
@@ -140,6 +146,17 @@ class Hierarchy {
   void vcycle_only(int precision, hipStream_t s);
   void set_graphs(bool on) { graphs_ = on; }
   bool graphs() const { return graphs_; }
+  // Measurement: `cycles` eager V-cycles on the current V-cycle input with HIP
+  // events between the phases of every level; ms[l] = the mean stream time of
+  // level l's own launches per cycle (its sweeps, residual, restriction and
+  // prolongation, or the coarsest solve; its children excluded), l < num_levels.
+  // Then `cycles` graph replays of the same cycle: *cycle_ms = their mean.
+  void level_times(int precision, int cycles, double* ms, double* cycle_ms, hipStream_t s);
+  // Algorithmic HBM bytes of one visit of level l in a zero-start cycle of the given
+  // precision, attributed like level_times: every operator and vector pass of the
+  // level's own launches, each array once (row pointers, column indices and values
+  // of A, R or P once per pass, the gathered vector once, the other vectors once).
+  double level_bytes(int l, int precision) const;
 
   // Stationary V-cycle iteration with physical-space input/output, the semantics of
   // ClassicalAMG.solve (classical_amg.py:1108-1155) and .apply (:1157-1180):
@@ -185,6 +202,7 @@ class Hierarchy {
   void build_coarsest(int l, hipStream_t s);
   void dense_inverse(Level& L, hipStream_t s);  // coarsest inverse on the matrix cores
   void refresh_f32(hipStream_t s);
+  void build_pad4(Level& L, hipStream_t s);
   void refresh_stencil(hipStream_t s);
   void alloc_vectors();
   // first_sweep_done: x already holds the first pre-smoothing sweep from zero
@@ -198,6 +216,18 @@ class Hierarchy {
   void capture_vcycle(int precision);
   int cycles_impl(const CycleWork& w, int precision, const double* b, double* x, bool use_x0, int min_cycles,
                   double tol, int max_cycles, double* rel, hipStream_t s);
+
+  // event marks of level_times (null outside it)
+  struct Clock {
+    std::vector<hipEvent_t> ev;
+    std::vector<std::array<int, 3>> seg;  // level, start mark, end mark
+    int used = 0;
+  };
+  Clock* clock_ = nullptr;
+  int mark(hipStream_t s);
+  void segment(int l, int m0, int m1) {
+    if (clock_) clock_->seg.push_back({l, m0, m1});
+  }
 
   bool graphs_ = true;
   bool transposed_ = false;  // eager cycles of vcycle64/32 with nu1 and nu2 swapped

@@ -207,6 +207,12 @@ struct Sel<double> {
   static void jac0(const double* b, const double* ir, double w, double* xo, int n, hipStream_t s) {
     launch_jacobi0_f64(b, ir, w, xo, n, s);
   }
+  static void jacL(const Level& L, const double* xi, const double* b, double w, double* xo, hipStream_t s) {
+    launch_jacobi_f64(A(L), xi, b, L.ir64.p, w, xo, L.group, s);
+  }
+  static void resL(const Level& L, const double* x, const double* b, double* r, hipStream_t s) {
+    launch_residual_f64(A(L), x, b, r, L.group, s);
+  }
   static void res(const CsrF64& A, const double* x, const double* b, double* r, int g, hipStream_t s) {
     launch_residual_f64(A, x, b, r, g, s);
   }
@@ -242,6 +248,17 @@ struct Sel<float> {
                   hipStream_t s) { launch_jacobi_f32(A, xi, b, ir, (float)w, xo, g, s); }
   static void jac0(const float* b, const float* ir, double w, float* xo, int n, hipStream_t s) {
     launch_jacobi0_f32(b, ir, (float)w, xo, n, s);
+  }
+  static Csr4F32 A4(const Level& L) {
+    return Csr4F32{L.n, L.nnz, L.rp4.p, reinterpret_cast<const int4*>(L.ci4.p), reinterpret_cast<const float4*>(L.v4.p)};
+  }
+  static void jacL(const Level& L, const float* xi, const float* b, double w, float* xo, hipStream_t s) {
+    if (L.pad4) launch_jacobi4_f32(A4(L), L.g4, xi, b, L.ir32.p, (float)w, xo, s);
+    else launch_jacobi_f32(A(L), xi, b, L.ir32.p, (float)w, xo, L.group, s);
+  }
+  static void resL(const Level& L, const float* x, const float* b, float* r, hipStream_t s) {
+    if (L.pad4) launch_residual4_f32(A4(L), L.g4, x, b, r, s);
+    else launch_residual_f32(A(L), x, b, r, L.group, s);
   }
   static void res(const CsrF32& A, const float* x, const float* b, float* r, int g, hipStream_t s) {
     launch_residual_f32(A, x, b, r, g, s);
@@ -643,10 +660,28 @@ void Hierarchy::dense_inverse(Level& L, hipStream_t s) {
   launch_dense_inverse_mfma(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.inv.p, L.gj_f.p, L.gj_p.p, L.gj_bad.p, s);
 }
 
+void Hierarchy::build_pad4(Level& L, hipStream_t s) {
+  DevBuf<int> cnt;
+  cnt.alloc(L.n);
+  L.rp4.alloc(L.n + 1);
+  launch_csr4_count(L.n, L.rp.p, cnt.p, s);
+  scan_i32_to_i32(cnt.p, L.rp4.p, L.n, s);
+  const long long nnz4 = read_scalar(L.rp4.p + L.n, s);
+  L.ci4.alloc(nnz4);
+  L.v4.alloc(nnz4);
+  launch_csr4_pattern(L.n, L.rp.p, L.ci.p, L.rp4.p, L.ci4.p, s);
+  launch_csr4_values(L.n, L.rp.p, L.v.p, L.rp4.p, L.v4.p, s);
+  L.g4 = csr4_group_for(L.n, nnz4);
+  L.pad4 = true;
+}
+
 void Hierarchy::refresh_f32(hipStream_t s) {
-  for (auto& L : levels_) {
+  for (int l = 0; l < (int)levels_.size(); ++l) {
+    Level& L = levels_[l];
     L.v32.alloc(L.nnz);
     launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    L.pad4 = false;
+    if (!(l == 0 && stencil_)) build_pad4(L, s);
     if (!L.coarsest) {
       L.pv32.alloc(L.pnnz);
       L.rv32.alloc(L.pnnz);
@@ -694,6 +729,7 @@ void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
     if (!(fine_done && l == 0)) {
       launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
       launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+      if (L.pad4) launch_csr4_values(L.n, L.rp.p, L.v.p, L.rp4.p, L.v4.p, s);
     }
     if (L.coarsest) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
@@ -760,7 +796,7 @@ void Hierarchy::smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& z
         launch_st7_jacobi_f32(S, (const float*)cur, (const float*)b, L.ir32.p, (float)prm_.omega, (float*)other, s);
       }
     } else {
-      Sel<T>::jac(Sel<T>::A(L), cur, b, Sel<T>::ir(L), prm_.omega, other, L.group, s);
+      Sel<T>::jacL(L, cur, b, prm_.omega, other, s);
     }
     std::swap(cur, other);
   }
@@ -788,11 +824,108 @@ void Hierarchy::coarse_solve(int l, const T* b, T* x, hipStream_t s) {
   if (cur != x) Sel<T>::copy(x, cur, L.n, s);
 }
 
+int Hierarchy::mark(hipStream_t s) {
+  if (!clock_) return -1;
+  Clock& c = *clock_;
+  if (c.used == (int)c.ev.size()) {
+    hipEvent_t e = nullptr;
+    IMPES_HIP_CHECK(hipEventCreate(&e));
+    c.ev.push_back(e);
+  }
+  IMPES_HIP_CHECK(hipEventRecord(c.ev[c.used], s));
+  return c.used++;
+}
+
+double Hierarchy::level_bytes(int l, int precision) const {
+  const Level& L = levels_.at(l);
+  const double vb = precision == 32 ? 4.0 : 8.0;
+  const double n = L.n;
+  // a CSR pass over `rows` rows with `nnz` entries gathering a vector of `cols`
+  // entries and streaming `vecs` more row vectors
+  auto csr = [&](double rows, double nnz, double cols, double vecs) {
+    return (rows + 1) * 4.0 + nnz * (4.0 + vb) + cols * vb + rows * vb * vecs;
+  };
+  const bool st = l == 0 && stencil_;
+  auto sweep = [&](double vecs) {  // L1-Jacobi (vecs 3: b, inv_relax, out) or residual (vecs 2: b, out)
+    return st ? n * vb * (4.0 + 1.0 + vecs) : csr(n, L.nnz, n, vecs);
+  };
+  if (L.coarsest) {
+    if (L.dense) return n * n * 8.0 + 2.0 * n * vb;
+    return 2.0 * n * vb + n * vb * 3.0 + (prm_.coarse_sweeps - 1) * sweep(3.0);
+  }
+  const Level& C = levels_[l + 1];
+  const bool first_done = l > 0 && prm_.pre_smooth > 0;  // fused into the parent's restriction
+  double b = 0.0;
+  int pre = prm_.pre_smooth - (first_done ? 1 : 0);
+  if (!first_done && pre > 0) {
+    b += 3.0 * n * vb;  // first sweep from zero: b, inv_relax -> x
+    --pre;
+  }
+  b += pre * sweep(3.0);
+  b += (prm_.pre_smooth == 0) ? 2.0 * n * vb : sweep(2.0);  // residual (a copy without pre-smoothing)
+  const bool fused = !C.coarsest && prm_.pre_smooth > 0;
+  b += csr(C.n, L.pnnz, n, fused ? 3.0 : 1.0);  // restriction (+ the coarse first sweep: inv_relax, b_c, x_c)
+  b += csr(n, L.pnnz, C.n, 2.0);                 // prolongation: x + P e_c
+  b += prm_.post_smooth * sweep(3.0);
+  if (prm_.post_smooth % 2) b += 2.0 * n * vb;  // the final copy back into x
+  return b;
+}
+
+void Hierarchy::level_times(int precision, int cycles, double* ms, double* cycle_ms, hipStream_t s) {
+  IMPES_REQUIRE(!levels_.empty(), "level_times before setup");
+  IMPES_REQUIRE(precision == 32 || precision == 64, "precision must be 32 or 64");
+  IMPES_REQUIRE(cycles >= 1, "cycles must be >= 1");
+  Clock c;
+  struct Done {
+    Hierarchy* h;
+    Clock& c;
+    ~Done() {
+      h->clock_ = nullptr;
+      for (auto e : c.ev) (void)hipEventDestroy(e);
+    }
+  } done{this, c};
+  Level& L0 = levels_[0];
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  clock_ = &c;
+  for (int i = 0; i < cycles; ++i) {
+    if (precision == 32) vcycle_level<float>(0, L0.b32.p, L0.x32.p, true, s);
+    else vcycle_level<double>(0, L0.b64.p, L0.x64.p, true, s);
+  }
+  clock_ = nullptr;
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<double> acc(levels_.size(), 0.0);
+  for (const auto& g : c.seg) {
+    float t = 0.f;
+    IMPES_HIP_CHECK(hipEventElapsedTime(&t, c.ev[g[1]], c.ev[g[2]]));
+    acc[g[0]] += t;
+  }
+  for (size_t l = 0; l < acc.size(); ++l) ms[l] = acc[l] / cycles;
+  if (cycle_ms) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    IMPES_HIP_CHECK(hipEventCreate(&e0));
+    IMPES_HIP_CHECK(hipEventCreate(&e1));
+    struct Ev {
+      hipEvent_t a, b;
+      ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } ev{e0, e1};
+    vcycle_only(precision, s);  // captures the graph when needed
+    IMPES_HIP_CHECK(hipEventRecord(e0, s));
+    for (int i = 0; i < cycles; ++i) vcycle_only(precision, s);
+    IMPES_HIP_CHECK(hipEventRecord(e1, s));
+    IMPES_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    IMPES_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *cycle_ms = t / cycles;
+  }
+}
+
 template <class T>
 void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s, bool first_sweep_done) {
   Level& L = levels_[l];
+  const int m0 = mark(s);
   if (L.coarsest) {
     coarse_solve<T>(l, b, x, s);
+    segment(l, m0, mark(s));
     return;
   }
   T* cur = x;
@@ -811,17 +944,19 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
       launch_st7_residual_f32(S, (const float*)cur, (const float*)b, (float*)r, s);
     }
   } else {
-    Sel<T>::res(Sel<T>::A(L), cur, b, r, L.group, s);
+    Sel<T>::resL(L, cur, b, r, s);
   }
   Level& C = levels_[l + 1];
-  if (!C.coarsest && prm_.pre_smooth > 0) {
+  const bool fused = !C.coarsest && prm_.pre_smooth > 0;
+  if (fused) {
     // restriction fused with the coarse level's first sweep from zero
     Sel<T>::rj0(Sel<T>::R(L), r, Sel<T>::ir(C), prm_.omega, Sel<T>::b(C), Sel<T>::x(C), L.rgroup, s);
-    vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s, true);
   } else {
     Sel<T>::spmv(Sel<T>::R(L), r, Sel<T>::b(C), L.rgroup, s);
-    vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s);
   }
+  segment(l, m0, mark(s));
+  vcycle_level<T>(l + 1, Sel<T>::b(C), Sel<T>::x(C), true, s, fused);
+  const int m1 = mark(s);
   // Post-smoothing alternates between x and t: prolongate into the buffer from
   // which post_smooth sweeps end in x, so the cycle needs no final copy.
   T* target = (prm_.post_smooth % 2 == 0) ? x : Sel<T>::t(L);
@@ -838,6 +973,7 @@ void Hierarchy::vcycle_level(int l, const T* b, T* x, bool x_zero, hipStream_t s
   other = (cur == x) ? Sel<T>::t(L) : x;
   smooth<T>(l, b, cur, other, prm_.post_smooth, zero, s);
   if (cur != x) Sel<T>::copy(x, cur, L.n, s);
+  segment(l, m1, mark(s));
 }
 
 namespace {

@@ -721,6 +721,35 @@ int impes_model_amg_levels(const impes_model* m, int* num_levels, int* sizes_hos
   });
 }
 
+static void level_times_into(Hierarchy& h, int precision, int cycles, double* ms_host, double* bytes_host,
+                             int max_levels, double* cycle_ms, hipStream_t s) {
+  IMPES_REQUIRE(ms_host && max_levels >= 0, "null pointer");
+  std::vector<double> ms(h.num_levels(), 0.0);
+  h.level_times(precision, cycles, ms.data(), cycle_ms, s);
+  for (int l = 0; l < h.num_levels() && l < max_levels; ++l) {
+    ms_host[l] = ms[l];
+    if (bytes_host) bytes_host[l] = h.level_bytes(l, precision);
+  }
+}
+
+int impes_model_amg_level_times(impes_model* m, int precision, int cycles, double* ms_host, double* bytes_host,
+                                int max_levels, double* cycle_ms, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(m, "null pointer");
+    Hierarchy* h = m->m.hierarchy_mut();
+    IMPES_REQUIRE(h && h->num_levels() > 0, "no pressure hierarchy yet (solve first)");
+    level_times_into(*h, precision, cycles, ms_host, bytes_host, max_levels, cycle_ms, as_stream(stream));
+  });
+}
+
+int impes_amg_level_times(impes_amg* amg, int precision, int cycles, double* ms_host, double* bytes_host,
+                          int max_levels, double* cycle_ms, void* stream) {
+  return guarded(__func__, [&] {
+    IMPES_REQUIRE(amg && amg->h.num_levels() > 0, "not set up");
+    level_times_into(amg->h, precision, cycles, ms_host, bytes_host, max_levels, cycle_ms, as_stream(stream));
+  });
+}
+
 int impes_model_amg_setup_seconds(const impes_model* m, double* out) {
   return guarded(__func__, [&] {
     IMPES_REQUIRE(m && out, "null pointer");

@@ -105,6 +105,7 @@ class Model {
   int num_wells() const { return nw_; }
   int report_len() const { return kReportHead + kWellSlots * nw_ + (n_env_ > 1 ? n_env_ : 0); }
   const Hierarchy* hierarchy() const { return amg_.get(); }
+  Hierarchy* hierarchy_mut() { return amg_.get(); }
 
   Grid grid;
   FluidParams F;

@@ -56,6 +56,29 @@ struct CsrF32 { int n_rows = 0, n_cols = 0, nnz = 0; const int* rp = nullptr; co
 template <class T>
 struct Stencil7 { int nx = 0, ny = 0, nz = 0; const T* d = nullptr; const T* ox = nullptr; const T* oy = nullptr; const T* oz = nullptr; };
 
+// FP32 CSR with every row padded to a multiple of 4 entries (value 0, column = the
+// row itself), 16-byte aligned: a lane reads 4 consecutive entries of its row with
+// one vector load, and the lanes of a wavefront read whole cache lines whatever
+// the row lengths. rp4 counts entries (multiples of 4); nnz is the unpadded count
+// (algorithmic bytes).
+struct Csr4F32 {
+  int n = 0, nnz = 0;
+  const int* rp4 = nullptr;
+  const int4* ci4 = nullptr;
+  const float4* v4 = nullptr;
+};
+// lanes per row of the padded kernels from the padded entry count
+int csr4_group_for(int n_rows, long long nnz4);
+// padded row lengths: cnt[r] = rows' lengths rounded up to 4
+void launch_csr4_count(int n, const int* rp, int* cnt, hipStream_t s);
+// padded pattern (ci4 sized rp4[n]) and FP32 values (rounded from FP64) of a CSR matrix
+void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s);
+void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s);
+// r = b - A x and xo = xi + w inv_relax .* (b - A xi) on the padded form
+void launch_residual4_f32(const Csr4F32& A, int group, const float* x, const float* b, float* r, hipStream_t s);
+void launch_jacobi4_f32(const Csr4F32& A, int group, const float* xi, const float* b, const float* inv_relax, float w,
+                        float* xo, hipStream_t s);
+
 // pick lanes-per-row from the mean row length
 int csr_group_for(int n_rows, int nnz);
 

@@ -10,6 +10,7 @@
 // so it is stored index-free as one diagonal and three face-coefficient arrays
 // (4 values per cell instead of CSR's ~7 values + 7 indices + row pointers).
 #include <cstdint>
+#include <cstdlib>
 #include <type_traits>
 
 #include "kernels.h"
@@ -64,6 +65,90 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
         y2[row] = w * oa * sum;
       }
     }
+  }
+}
+
+// Padded FP32 CSR (Csr4F32): G lanes per row, each lane 4 entries per 16-byte load.
+template <int G, int MODE>
+__global__ __launch_bounds__(kBlock) void k_csr4(int n, const int* __restrict__ rp4, const int4* __restrict__ ci4,
+                                                  const float4* __restrict__ v4, const float* __restrict__ x,
+                                                  const float* __restrict__ b, const float* __restrict__ aux, float w,
+                                                  float* __restrict__ y) {
+  constexpr int RPB = kBlock / G;
+  const int g = threadIdx.x % G;
+  for (long long base = xcd_block() * RPB; base < n; base += (long long)gridDim.x * RPB) {
+    const long long row = base + threadIdx.x / G;
+    const bool active = row < n;
+    // the row's own operands are loaded before the row walk (as in k_csr)
+    float ob = 0.f, ox = 0.f, oa = 0.f;
+    if (active && g == 0) {
+      ob = b[row];
+      if (MODE == kJac) {
+        ox = x[row];
+        oa = aux[row];
+      }
+    }
+    float sum = 0.f;
+    if (active) {
+      const int q1 = rp4[row + 1] >> 2;
+#pragma unroll 2
+      for (int q = (rp4[row] >> 2) + g; q < q1; q += G) {
+        const float4 a = v4[q];
+        const int4 c = ci4[q];
+        sum += a.x * x[c.x];
+        sum += a.y * x[c.y];
+        sum += a.z * x[c.z];
+        sum += a.w * x[c.w];
+      }
+    }
+    if (G > 1) sum = group_sum<G>(sum);
+    if (active && g == 0) {
+      if (MODE == kRes) y[row] = ob - sum;
+      else y[row] = ox + w * oa * (ob - sum);
+    }
+  }
+}
+
+template <int MODE>
+void csr4_dispatch(const Csr4F32& A, int G, const float* x, const float* b, const float* aux, float w, float* y,
+                   hipStream_t s) {
+  if (A.n <= 0) return;
+  IMPES_REQUIRE(((uintptr_t)A.ci4 & 15) == 0 && ((uintptr_t)A.v4 & 15) == 0, "padded CSR: arrays not 16-byte aligned");
+  const int grid = grid_for(A.n, kBlock / G, 1 << 18);
+  switch (G) {
+    case 1: klaunch(k_csr4<1, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    case 2: klaunch(k_csr4<2, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    case 4: klaunch(k_csr4<4, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    case 8: klaunch(k_csr4<8, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    case 16: klaunch(k_csr4<16, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    case 32: klaunch(k_csr4<32, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+    default: klaunch(k_csr4<64, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+  }
+  IMPES_LAUNCH_CHECK();
+}
+
+__global__ void k_csr4_count(int n, const int* __restrict__ rp, int* __restrict__ cnt) {
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x)
+    cnt[r] = (rp[r + 1] - rp[r] + 3) & ~3;
+}
+
+__global__ void k_csr4_pattern(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const int* __restrict__ rp4, int* __restrict__ ci4) {
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
+    const int e0 = rp[r], len = rp[r + 1] - e0, q0 = rp4[r], len4 = rp4[r + 1] - q0;
+    for (int k = 0; k < len4; ++k) ci4[q0 + k] = (k < len) ? ci[e0 + k] : (int)r;
+  }
+}
+
+// G lanes per row (entries strided by G): consecutive lanes write consecutive entries
+template <int G>
+__global__ void k_csr4_values(int n, const int* __restrict__ rp, const double* __restrict__ v,
+                              const int* __restrict__ rp4, float* __restrict__ v4) {
+  const int g = threadIdx.x % G;
+  for (long long r = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / G; r < n;
+       r += (long long)gridDim.x * blockDim.x / G) {
+    const int e0 = rp[r], len = rp[r + 1] - e0, q0 = rp4[r], len4 = rp4[r + 1] - q0;
+    for (int k = g; k < len4; k += G) v4[q0 + k] = (k < len) ? (float)v[e0 + k] : 0.f;
   }
 }
 
@@ -398,6 +483,40 @@ void launch_jacobi_f32(const CsrF32& A, const float* xi, const float* b, const f
                        int G, hipStream_t s) {
   KScope ks("csr_jacobi_f32", csr_bytes(A.n_rows, A.nnz, 4, 3), s);
   csr_dispatch<float, float, kJac>(A.n_rows, A.rp, A.ci, A.v, xi, b, inv_relax, w, xo, G, s);
+}
+int csr4_group_for(int n_rows, long long nnz4) {
+  if (n_rows <= 0) return 1;
+  // at most one 4-entry chunk per lane on a row of mean length: the smallest power
+  // of two not below the mean number of chunks (measured on the 1M-cell IMPES
+  // hierarchy: 8 lanes on level 1 with ~4 chunks per row beat 4 lanes by 7%)
+  const double chunks = (double)nnz4 / 4.0 / (double)n_rows;
+  int g = 1;
+  while (g < 64 && g < chunks) g <<= 1;
+  return g;
+}
+void launch_csr4_count(int n, const int* rp, int* cnt, hipStream_t s) {
+  if (n <= 0) return;
+  k_csr4_count<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, cnt);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s) {
+  if (n <= 0) return;
+  k_csr4_pattern<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, rp4, ci4);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s) {
+  if (n <= 0) return;
+  k_csr4_values<8><<<grid_for((long long)n * 8, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, v, rp4, v4);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_residual4_f32(const Csr4F32& A, int G, const float* x, const float* b, float* r, hipStream_t s) {
+  KScope ks("csr_residual_f32", csr_bytes(A.n, A.nnz, 4, 2), s);
+  csr4_dispatch<kRes>(A, G, x, b, nullptr, 0.f, r, s);
+}
+void launch_jacobi4_f32(const Csr4F32& A, int G, const float* xi, const float* b, const float* inv_relax, float w,
+                        float* xo, hipStream_t s) {
+  KScope ks("csr_jacobi_f32", csr_bytes(A.n, A.nnz, 4, 3), s);
+  csr4_dispatch<kJac>(A, G, xi, b, inv_relax, w, xo, s);
 }
 void launch_restrict_j0_f64(const CsrF64& R, const double* r, const double* inv_relax_c, double w, double* bc,
                             double* xc, int G, hipStream_t s) {

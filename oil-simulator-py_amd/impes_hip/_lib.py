@@ -121,6 +121,7 @@ SIGNATURES = {
     "impes_amg_num_levels": [_P, C.POINTER(_I)],
     "impes_amg_level_info": [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)],
     "impes_amg_equilibrated": [_P, C.POINTER(_I)],
+    "impes_amg_level_times": [_P, _I, _I, _PD, _PD, _I, _PD, _P],
     "impes_amg_setup_seconds": [_P, C.POINTER(_D)],
     "impes_amg_export_level": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "impes_amg_export_scaling": [_P, _P, _P],
@@ -160,6 +161,7 @@ SIGNATURES = {
     "impes_model_eval_pvt": [_P, _I, _L, _P, _P, _P],
     "impes_model_amg_levels": [_P, _PI, _PI, _PI, _I],
     "impes_model_amg_setup_seconds": [_P, _PD],
+    "impes_model_amg_level_times": [_P, _I, _I, _PD, _PD, _I, _PD, _P],
     "impes_ktimer_config": [C.c_char_p, _I],
     "impes_ktimer_read": [_PD, _PD, C.POINTER(_L)],
 }

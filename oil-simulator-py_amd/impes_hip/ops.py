@@ -177,6 +177,16 @@ class AmgHierarchy:
         return {"n": n.value, "nnz": nnz.value, "p_nnz": pnnz.value, "is_coarsest": bool(co.value),
                 "dense": bool(de.value)}
 
+    def level_times(self, precision=32, cycles=20):
+        """Per-level V-cycle figures on the hierarchy's current V-cycle input
+        (impes_amg_level_times): (ms per eager cycle of each level's own launches,
+        their algorithmic bytes per visit, ms of one graph-replayed cycle)."""
+        ms, by = (C.c_double * 64)(), (C.c_double * 64)()
+        cyc = C.c_double()
+        call("impes_amg_level_times", self._h, int(precision), int(cycles), ms, by, 64, C.byref(cyc), stream_ptr())
+        nl = self.num_levels
+        return [ms[i] for i in range(nl)], [by[i] for i in range(nl)], cyc.value
+
     @property
     def equilibrated(self) -> bool:
         v = C.c_int()
@@ -449,6 +459,17 @@ class ImpesModel:
         nnz = (C.c_int * max_levels)()
         call("impes_model_amg_levels", self._h, C.byref(nl), sizes, nnz, max_levels)
         return [(sizes[i], nnz[i]) for i in range(min(nl.value, max_levels))]
+
+    def amg_level_times(self, precision=32, cycles=20):
+        """Per-level V-cycle figures of the pressure hierarchy
+        (impes_model_amg_level_times): (ms per eager cycle of each level's own
+        launches, their algorithmic bytes per visit, ms of one graph-replayed cycle)."""
+        ms, by = (C.c_double * 64)(), (C.c_double * 64)()
+        cyc = C.c_double()
+        call("impes_model_amg_level_times", self._h, int(precision), int(cycles), ms, by, 64, C.byref(cyc),
+             stream_ptr())
+        nl = len(self.amg_levels())
+        return [ms[i] for i in range(nl)], [by[i] for i in range(nl)], cyc.value
 
     def amg_setup_seconds(self) -> float:
         """Wall time of the last hierarchy build (0.0 before the first solve)."""

@@ -184,6 +184,35 @@ def test_coarsest_level_edge_cases(dev, case):
     assert rel(hg.vcycle(t(b, dev), precision=32).cpu().numpy(), xo) < 1e-4
 
 
+@pytest.mark.parametrize("setup", ["csr", "stencil"])
+def test_level_times_cover_every_level(dev, setup):
+    """impes_amg_level_times: a positive, finite time and byte count for every
+    level in both precisions, bytes at least the level's four operator passes
+    (V(2,2) with the first sweep fused into the restriction above it), a positive
+    graph-replayed cycle time, and the preconditioner unchanged by the timing
+    cycles."""
+    import impes_hip as H
+
+    dims = (20, 20, 10)
+    A, d, ox, oy, oz = impes_like(*dims, seed=1)
+    hg = H.AmgHierarchy()
+    if setup == "csr":
+        hg.setup_csr(*to_dev(A, dev))
+    else:
+        hg.setup_st7(dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev))
+    v = t(np.random.default_rng(3).standard_normal(A.shape[0]), dev)
+    z0 = hg.precondition(v, precision=32).clone()
+    for prec in (32, 64):
+        ms, by, cyc = hg.level_times(precision=prec, cycles=3)
+        assert len(ms) == len(by) == hg.num_levels
+        assert all(np.isfinite(m) and m > 0 for m in ms) and cyc > 0
+        vb = 4 if prec == 32 else 8
+        for l in range(1, hg.num_levels - 1):
+            assert by[l] >= 4 * hg.level_info(l)["nnz"] * (4 + vb), l
+        assert all(b > 0 for b in by)
+    assert torch.equal(hg.precondition(v, precision=32), z0)
+
+
 def test_hierarchy_matches_oracle_131k_rows(dev):
     """131k rows: coarse levels with 40-70 entries per row (the setup's
     wavefront-per-row kernels) and product lists of up to ~4200 entries per row
