@@ -79,7 +79,8 @@ typedef struct {
   double omega;          /* L1-Jacobi damping (default 0.8; reference 0.7) */
   double equil_threshold;/* equilibrate when the median |a_ii| < threshold (1e-6) */
   int dense_max;         /* coarsest level solved densely up to this size (500; at most 4096: above
-                          * 512 rows the inverse is formed on the matrix cores, MFMA FP64) */
+                          * 160 rows the inverse is formed on the matrix cores, MFMA FP64, when
+                          * the level is SPD; else by pivoted Gauss-Jordan up to 512 rows) */
   int coarse_sweeps;     /* Jacobi sweeps on a larger coarsest level (50) */
   int anchor;            /* forced C point on every level, -1 = none (reference anchor_idx) */
   /* Preconditioner choices that are not in the reference; they change the iteration

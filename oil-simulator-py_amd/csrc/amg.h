@@ -44,8 +44,9 @@ struct AmgParams {
   int post_smooth = 2;
   double omega = 0.8;
   double equil_threshold = 1e-6;  // equilibrate when median |a_ii| is below this
-  int dense_max = 500;            // largest coarsest level solved densely (<= 4096; above 512 rows
-                                  // the inverse is formed on the matrix cores)
+  int dense_max = 500;            // largest coarsest level solved densely (<= 4096; above 160 rows
+                                  // the inverse is formed on the matrix cores when the level is
+                                  // SPD, else by pivoted Gauss-Jordan up to 512 rows)
   int coarse_sweeps = 50;         // Jacobi sweeps on a coarsest level above dense_max
   int anchor = -1;                // forced C point on every level (reference anchor_idx), -1 = none
   // Additions that change the preconditioner (not the converged answer).
@@ -97,7 +98,7 @@ struct Level {
   DevBuf<float> x32, b32, r32, t32;
   // coarsest level: [A_pinned^-1] stored in the right half of an n x 2n matrix
   DevBuf<double> aug;
-  // coarsest level above 512 rows: A_pinned^-1 padded to dense_pad(n) (matrix cores)
+  // coarsest level above 160 rows: A_pinned^-1 padded to dense_pad(n) (matrix cores)
   bool mfma = false;
   DevBuf<double> inv, gj_f, gj_p;
   DevBuf<int> gj_bad;

@@ -638,7 +638,9 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
   L.pin.alloc(2);  // [0] pin flag, [1] inverse produced by the SPD sweep
   launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);
   if (!L.dense) return;
-  L.mfma = L.n > 512;
+  // above 160 rows the blocked matrix-core inverse is faster than the one-workgroup
+  // sweep (a 508-row coarsest level: 0.3 ms per refresh against 85 ms)
+  L.mfma = L.n > kDenseSweepMax;
   if (L.mfma) {
     const long long np = dense_pad(L.n);
     L.inv.alloc(np * np);
@@ -646,9 +648,15 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
     L.gj_p.alloc(64 * 64);
     L.gj_bad.alloc(1);
     dense_inverse(L, s);
-    IMPES_REQUIRE(read_scalar(L.gj_bad.p, s) == 0,
-                  "amg setup: coarsest operator is not symmetric positive definite (lower dense_max)");
-    return;
+    if (read_scalar(L.gj_bad.p, s) == 0) return;
+    // a non-positive pivot: not SPD; up to 512 rows the pivoted elimination below
+    IMPES_REQUIRE(L.n <= 512, "amg setup: coarsest operator above 512 rows is not symmetric positive definite "
+                              "(lower dense_max)");
+    L.mfma = false;
+    L.inv.release();
+    L.gj_f.release();
+    L.gj_p.release();
+    L.gj_bad.release();
   }
   L.aug.alloc(2LL * L.n * L.n);
   launch_csr_to_dense(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.aug.p, s);

@@ -12,17 +12,27 @@ namespace impes {
 
 namespace {
 
-// One workgroup: pin = mode==1 || (mode==2 && no row has |row sum| > 1e-8 |a_ii|).
-__global__ __launch_bounds__(256) void k_pin_decide(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                    const double* __restrict__ v, int mode, int* __restrict__ pin) {
+// One workgroup, 8 lanes per row: pin = mode==1 || (mode==2 && no row has
+// |row sum| > 1e-8 |a_ii|).
+constexpr int kPinThreads = 1024;
+constexpr int kPinLanes = 8;
+__global__ __launch_bounds__(kPinThreads) void k_pin_decide(int n, const int* __restrict__ rp,
+                                                            const int* __restrict__ ci, const double* __restrict__ v,
+                                                            int mode, int* __restrict__ pin) {
+  const int g = threadIdx.x % kPinLanes;
   int regular = 0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+  for (int base = 0; base < n; base += kPinThreads / kPinLanes) {
+    const int r = base + (int)threadIdx.x / kPinLanes;
     double rs = 0.0, d = 0.0;
-    for (int e = rp[r]; e < rp[r + 1]; ++e) {
-      rs += v[e];
-      if (ci[e] == r) d += v[e];
+    if (r < n) {
+      for (int e = rp[r] + g; e < rp[r + 1]; e += kPinLanes) {
+        rs += v[e];
+        if (ci[e] == r) d += v[e];
+      }
     }
-    if (fabs(rs) > 1e-8 * fabs(d)) regular = 1;
+    rs = group_sum<kPinLanes>(rs);
+    d = group_sum<kPinLanes>(d);
+    if (r < n && g == 0 && fabs(rs) > 1e-8 * fabs(d)) regular = 1;
   }
   regular = __syncthreads_or(regular);
   if (threadIdx.x == 0) *pin = (mode == 1) ? 1 : (mode == 2 ? !regular : 0);
@@ -375,7 +385,7 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const int* __restric
 }  // namespace
 
 void launch_pin_decide(int n, const int* rp, const int* ci, const double* v, int mode, int* pin, hipStream_t s) {
-  k_pin_decide<<<1, 256, 0, s>>>(n, rp, ci, v, mode, pin);
+  k_pin_decide<<<1, kPinThreads, 0, s>>>(n, rp, ci, v, mode, pin);
   IMPES_LAUNCH_CHECK();
 }
 

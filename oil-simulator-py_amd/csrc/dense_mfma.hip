@@ -7,13 +7,11 @@
 // matrix and itself SPD: no pivoting is needed.
 //
 // The matrix (n x n, row-major) is padded to n_pad = 64 nb with an identity block.
-// For each block column k (one step of Gauss-Jordan on 64 columns at once):
-//   P = A_kk^-1                          (one workgroup, in LDS)
-//   A_kj <- P A_kj        (j != k)       row panel, MFMA
-//   F    <- A_ik          (i != k)       copy of the old column panel
-//   A_ij <- A_ij - F_i A_kj (i, j != k)  trailing update, MFMA
-//   A_ik <- -F_i P        (i != k)       MFMA
-//   A_kk <- P
+// For each block column k (one step of Gauss-Jordan on 64 columns at once), three
+// launches:
+//   P = A_kk^-1                          one workgroup, the block in registers
+//   F_i <- A_ik (all i); A_kj <- P A_kj (j != k, MFMA); A_kk <- P
+//   A_ij <- A_ij - F_i A_kj (i, j != k); A_ik <- -F_i P (i != k)   MFMA
 // which is the scalar in-place elimination (a_kj /= a_kk; a_ij -= a_ik a_kj;
 // a_ik = -a_ik / a_kk; a_kk = 1 / a_kk) on 64 x 64 blocks; after the last block A
 // holds A^-1. Each visit of the V-cycle then applies the inverse as one dense
@@ -91,41 +89,65 @@ __global__ __launch_bounds__(256) void k_fill_padded(int n, int n_pad, const int
   for (int c = threadIdx.x; c < n_pad; c += blockDim.x) M[(long long)r * n_pad + c] = row[c];
 }
 
-// P = A_kk^-1 by unpivoted Gauss-Jordan in LDS (one workgroup). A non-positive
-// pivot (not SPD) sets *bad.
+// P = A_kk^-1 by unpivoted Gauss-Jordan with the 64 x 64 block in registers: thread
+// t holds column j = t % 64 at rows i = t / 64 + 4 m (m < 16). Each pivot p
+// exchanges only the old pivot column and the old pivot row through LDS
+// (double-buffered: one barrier per pivot), then every thread updates its 16
+// entries with the scalar elimination's formulas:
+//   row p: a_pj / a_pp (a_pp -> 1 / a_pp); column p: -a_ip / a_pp; else a_ij - a_ip (a_pj / a_pp).
+// A non-positive pivot (not SPD) sets *bad.
 __global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double* __restrict__ M,
                                                  double* __restrict__ P, int* __restrict__ bad) {
-  __shared__ double a[kB * kLd];
-  __shared__ double col[kB];
-  const long long off = (long long)k * kB * n_pad + (long long)k * kB;
-  load_tile(M + off, n_pad, a);
-  __syncthreads();
+  __shared__ double colb[2][kB], rowb[2][kB];
+  const int j = threadIdx.x & (kB - 1), i0 = threadIdx.x >> 6;
+  const double* src = M + (long long)k * kB * n_pad + (long long)k * kB;
+  double a[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) a[m] = src[(long long)(i0 + 4 * m) * n_pad + j];
   for (int p = 0; p < kB; ++p) {
-    const double piv = a[p * kLd + p];
-    if (threadIdx.x == 0 && !(piv > 0.0)) *bad = 1;
-    // stage column p (old), then scale row p with a_pp = 1 replaced first
-    for (int i = threadIdx.x; i < kB; i += blockDim.x) col[i] = a[i * kLd + p];
-    __syncthreads();
-    const double inv = 1.0 / piv;
-    for (int j = threadIdx.x; j < kB; j += blockDim.x) a[p * kLd + j] = ((j == p) ? 1.0 : a[p * kLd + j]) * inv;
-    __syncthreads();
-    for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
-      const int i = q / kB, j = q % kB;
-      if (i == p) continue;
-      const double f = col[i];
-      a[i * kLd + j] = ((j == p) ? 0.0 : a[i * kLd + j]) - f * a[p * kLd + j];
+    const int buf = p & 1;
+    if (j == p) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) colb[buf][i0 + 4 * m] = a[m];
+    }
+    if (i0 == (p & 3)) {
+      const int mp = p >> 2;
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v = (m == mp) ? a[m] : v;
+      rowb[buf][j] = v;
     }
     __syncthreads();
+    const double piv = rowb[buf][p];
+    if (threadIdx.x == 0 && !(piv > 0.0)) *bad = 1;
+    const double inv = 1.0 / piv;
+    const double r = (j == p) ? inv : rowb[buf][j] * inv;  // the new row p
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = i0 + 4 * m;
+      const double c = colb[buf][i];
+      a[m] = (i == p) ? r : ((j == p) ? -c * inv : a[m] - c * r);
+    }
   }
-  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) P[q] = a[(q / kB) * kLd + q % kB];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) P[(i0 + 4 * m) * kB + j] = a[m];
 }
 
-// Row panel: A_kj <- P A_kj for every column block j != k (one workgroup each).
+// Row panel, one workgroup per block index j: first F_j = A_jk (the old column
+// panel, which no workgroup of this launch writes), then A_kj <- P A_kj for j != k
+// and A_kk <- P.
 __global__ __launch_bounds__(kGemmThreads) void k_gj_row(int k, int n_pad, double* __restrict__ M,
-                                                         const double* __restrict__ P) {
+                                                         const double* __restrict__ P, double* __restrict__ F) {
   __shared__ double As[kB * kLd], Bs[kB * kLd];
   const int j = blockIdx.x;
-  if (j == k) return;
+  const double* colblk = M + (long long)j * kB * n_pad + (long long)k * kB;
+  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) F[(long long)j * kB * kB + q] = colblk[(q / kB) * (long long)n_pad + q % kB];
+  if (j == k) {
+    __syncthreads();  // the copy of A_kk above is read before it is overwritten
+    for (int q = threadIdx.x; q < kB * kB; q += blockDim.x)
+      M[(long long)(k * kB + q / kB) * n_pad + (long long)k * kB + q % kB] = P[q];
+    return;
+  }
   double* blk = M + (long long)k * kB * n_pad + (long long)j * kB;
   load_tile(P, kB, As);
   load_tile(blk, n_pad, Bs);
@@ -140,16 +162,6 @@ __global__ __launch_bounds__(kGemmThreads) void k_gj_row(int k, int n_pad, doubl
       tile_pos(t, e, r, c);
       blk[(long long)r * n_pad + c] = acc[t][e];
     }
-}
-
-// F = column panel k (n_pad x 64), before the trailing update overwrites it.
-__global__ void k_gj_col(int k, int n_pad, const double* __restrict__ M, double* __restrict__ F) {
-  const long long total = (long long)n_pad * kB;
-  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total; q += (long long)gridDim.x * blockDim.x) {
-    const long long r = q / kB;
-    const int c = (int)(q % kB);
-    F[q] = M[r * n_pad + (long long)k * kB + c];
-  }
 }
 
 // Trailing update of block (i, j), i != k: A_ij -= F_i A_kj (j != k) or
@@ -178,12 +190,8 @@ __global__ __launch_bounds__(kGemmThreads) void k_gj_update(int k, int n_pad, do
     }
 }
 
-__global__ void k_gj_put_diag(int k, int n_pad, double* __restrict__ M, const double* __restrict__ P) {
-  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x)
-    M[(long long)(k * kB + q / kB) * n_pad + (long long)k * kB + q % kB] = P[q];
-}
-
-// x = A^-1 b (b_0 taken as 0 when DOF 0 is pinned): a wavefront per row.
+// x = A^-1 b (b_0 taken as 0 when DOF 0 is pinned): a wavefront per row, four
+// independent partial sums per lane (four loads in flight), folded at the end.
 template <class T>
 __global__ __launch_bounds__(256) void k_dense_mv(int n, int n_pad, const int* __restrict__ pin_flag,
                                                   const double* __restrict__ M, const T* __restrict__ b,
@@ -194,9 +202,16 @@ __global__ __launch_bounds__(256) void k_dense_mv(int n, int n_pad, const int* _
   const int nwaves = (gridDim.x * blockDim.x) / kWave;
   for (int r = wid; r < n; r += nwaves) {
     const double* row = M + (long long)r * n_pad;
-    double s = 0.0;
-    for (int c = (pin ? 1 : 0) + lane; c < n; c += kWave) s += row[c] * (double)b[c];
-    s = wave_sum(s);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = (pin ? 1 : 0) + lane;
+    for (; c + 3 * kWave < n; c += 4 * kWave) {
+      s0 += row[c] * (double)b[c];
+      s1 += row[c + kWave] * (double)b[c + kWave];
+      s2 += row[c + 2 * kWave] * (double)b[c + 2 * kWave];
+      s3 += row[c + 3 * kWave] * (double)b[c + 3 * kWave];
+    }
+    for (; c < n; c += kWave) s0 += row[c] * (double)b[c];
+    const double s = wave_sum((s0 + s1) + (s2 + s3));
     if (lane == 0) x[r] = (T)s;
   }
 }
@@ -214,13 +229,9 @@ void launch_dense_inverse_mfma(int n, const int* rp, const int* ci, const double
   for (int k = 0; k < nb; ++k) {
     k_gj_diag<<<1, 256, 0, s>>>(k, n_pad, M, P, bad);
     IMPES_LAUNCH_CHECK();
-    k_gj_row<<<nb, kGemmThreads, 0, s>>>(k, n_pad, M, P);
-    IMPES_LAUNCH_CHECK();
-    k_gj_col<<<div_up(n_pad * kB, 256), 256, 0, s>>>(k, n_pad, M, F);
+    k_gj_row<<<nb, kGemmThreads, 0, s>>>(k, n_pad, M, P, F);
     IMPES_LAUNCH_CHECK();
     k_gj_update<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, M, F, P);
-    IMPES_LAUNCH_CHECK();
-    k_gj_put_diag<<<1, 256, 0, s>>>(k, n_pad, M, P);
     IMPES_LAUNCH_CHECK();
   }
 }

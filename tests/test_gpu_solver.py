@@ -145,23 +145,28 @@ def test_hierarchy_matches_oracle(dev, case, mode):
 
 
 @pytest.mark.parametrize("case", ["max_levels_1", "max_levels_2", "single_level_swept", "single_level_dense",
-                                  "mfma_2342", "mfma_695", "mfma_1346_pinned", "mfma_single_4000"])
+                                  "single_level_dense_negdef", "mfma_2342", "mfma_695", "mfma_1346_pinned",
+                                  "mfma_single_4000"])
 def test_coarsest_level_edge_cases(dev, case):
     """Coarsest levels above the dense limit (500 rows: solved by 50 L1-Jacobi
     sweeps, classical_amg.py:1045-1069) after one or two coarsenings, and
-    one-level hierarchies (no pin) solved by sweeps or densely; dense coarsest
-    levels of 695 to 4000 rows (dense_max 4096), whose inverse is formed on the
-    matrix cores (blocked Gauss-Jordan, MFMA FP64), with and without the DOF-0
-    pin (the reference's hierarchy pins unconditionally): hierarchy and V-cycle
-    against the oracle (numpy's dense inverse)."""
+    one-level hierarchies (no pin) solved by sweeps or densely (400 rows: the
+    matrix-core inverse, and for the negated, negative definite operator its
+    fallback to pivoted Gauss-Jordan); dense coarsest levels of 695 to 4000 rows
+    (dense_max 4096), whose inverse is formed on the matrix cores (blocked
+    Gauss-Jordan, MFMA FP64), with and without the DOF-0 pin (the reference's
+    hierarchy pins unconditionally): hierarchy and V-cycle against the oracle
+    (numpy's dense inverse)."""
     import impes_hip as H
 
     mfma = {"mfma_2342": ((40, 40, 10), dict(coarsest_size=3000, dense_max=4096)),
             "mfma_695": ((40, 40, 10), dict(coarsest_size=1500, dense_max=4096)),
             "mfma_1346_pinned": ((30, 30, 12), dict(coarsest_size=1500, dense_max=4096, **oa.Hierarchy.REFERENCE)),
             "mfma_single_4000": ((20, 20, 10), dict(coarsest_size=5000, dense_max=4096))}
-    if case == "single_level_dense":
+    if case in ("single_level_dense", "single_level_dense_negdef"):
         A, *_ = impes_like(10, 8, 5, seed=4)
+        if case == "single_level_dense_negdef":  # not SPD: the matrix-core inverse falls back to pivoting
+            A = -A
         kw = dict(coarsest_size=1000)
     elif case in mfma:
         dims, kw = mfma[case]
@@ -175,7 +180,7 @@ def test_coarsest_level_edge_cases(dev, case):
     hg.setup_csr(*to_dev(A, dev))
     _compare_hierarchy(hg, ho)
     info = hg.level_info(hg.num_levels - 1)
-    assert bool(info["dense"]) == (case == "single_level_dense" or case in mfma)
+    assert bool(info["dense"]) == (case.startswith("single_level_dense") or case in mfma)
     if case in mfma:
         assert info["n"] > 512 and ho.levels[-1]["pin"] == (case == "mfma_1346_pinned")
     b = np.random.default_rng(5).standard_normal(A.shape[0])
