@@ -1,6 +1,8 @@
 // Restarted flexible GMRES in FP64 with classical Gram-Schmidt and a second,
-// always-run orthogonalisation pass (CGS2), right-preconditioned by the AMG
-// V-cycle (FP64 or FP32 hierarchy). The Krylov method runs on the row-scaled system
+// always-run orthogonalisation pass (CGS2; with a preconditioner and restart <= 32
+// the second pass of each basis vector is lagged into the next Arnoldi step, two
+// passes over the basis per step instead of three), right-preconditioned by the
+// AMG V-cycle (FP64 or FP32 hierarchy). The Krylov method runs on the row-scaled system
 // D^-1 A x = D^-1 b (D = diag A): its residual is a per-cell pressure
 // correction in Pa, so "relative residual" compares like with like even when
 // well rows exceed interior rows by 15 orders of magnitude.
@@ -56,6 +58,7 @@ class Fgmres {
   long long n_ = 0;
   int m_ = 0;
   DevBuf<double> V_, Z_, w_, bs_, small_, work_;
+  bool cgs2_ = false;           // IMPES_FGMRES_CGS2=1: three-pass CGS2 also where the lagged form applies
   double* host_ = nullptr;      // pinned scalars: [0] blocking reads, [2..3] per-iteration estimates
   double* host_dev_ = nullptr;  // device address of host_ (the estimate is stored there by the device)
   hipEvent_t est_ev_[2] = {nullptr, nullptr};

@@ -1,7 +1,7 @@
-// FP64 restarted FGMRES(m) with CGS2 orthogonalisation. The Hessenberg matrix,
-// Givens rotations and the residual estimate live on the device; the host reads
-// one scalar per Arnoldi step (written by the device into pinned host memory) to
-// decide termination.
+// FP64 restarted FGMRES(m) with CGS2 orthogonalisation (lagged by one step where a
+// preconditioner is used and m <= 32). The Hessenberg matrix, Givens rotations and
+// the residual estimate live on the device; the host reads one scalar per Arnoldi
+// step (written by the device into pinned host memory) to decide termination.
 //
 // One Arnoldi step with the AMG preconditioner is, besides the V-cycle:
 //   st7/CSR operator apply (fused with the V-cycle's output scaling on stencils),
@@ -32,7 +32,11 @@ struct Layout {
   __host__ __device__ int h2() const { return h() + m + 1; }
   __host__ __device__ int y() const { return h2() + m + 1; }
   __host__ __device__ int sc() const { return y() + m; }  // 8 scalars
-  __host__ __device__ int size() const { return sc() + 8; }
+  // lagged orthogonalisation: provisional column (m+1) and the update's 2 scalars;
+  // its coefficients a and c reuse h() and h2()
+  __host__ __device__ int hp() const { return sc() + 8; }
+  __host__ __device__ int lag() const { return hp() + m + 1; }
+  __host__ __device__ int size() const { return lag() + 2; }
 };
 
 // sc[0] = beta = sqrt(sc[4]) ; g = beta e1
@@ -160,16 +164,23 @@ __global__ void k_fold_cols(const double* __restrict__ part, int nb, double* __r
   if (threadIdx.x == 0) out[c] = s;
 }
 
+// Workgroups of a Gram-Schmidt pass: each folds its rows' k products per column
+// with wavefront shuffles, so more rows per thread amortise the folds (1024: 1%
+// faster per time step than 2048 at 1M cells; 512 and 256 no better).
+constexpr int kGsBlocks = 1024;
+static_assert(kGsBlocks <= kMultiDotBlocks, "the partials workspace is sized for kMultiDotBlocks");
+
 template <int PASS>
 void launch_gs_pass(int k, double* w, const double* V, long long n, const double* h, double* part, hipStream_t s) {
+  const int nb = kGsBlocks;
   if (k <= 4)
-    k_gs_pass<4, PASS><<<kMultiDotBlocks, kGsThreads, 0, s>>>(k, w, V, n, h, part);
+    k_gs_pass<4, PASS><<<nb, kGsThreads, 0, s>>>(k, w, V, n, h, part);
   else if (k <= 8)
-    k_gs_pass<8, PASS><<<kMultiDotBlocks, kGsThreads, 0, s>>>(k, w, V, n, h, part);
+    k_gs_pass<8, PASS><<<nb, kGsThreads, 0, s>>>(k, w, V, n, h, part);
   else if (k <= 16)
-    k_gs_pass<16, PASS><<<kMultiDotBlocks, kGsThreads, 0, s>>>(k, w, V, n, h, part);
+    k_gs_pass<16, PASS><<<nb, kGsThreads, 0, s>>>(k, w, V, n, h, part);
   else
-    k_gs_pass<kMaxFusedGs, PASS><<<kMultiDotBlocks, kGsThreads, 0, s>>>(k, w, V, n, h, part);
+    k_gs_pass<kMaxFusedGs, PASS><<<nb, kGsThreads, 0, s>>>(k, w, V, n, h, part);
   IMPES_LAUNCH_CHECK();
 }
 
@@ -181,6 +192,273 @@ __global__ void k_arnoldi(Layout L, int j, double* __restrict__ sm, const double
   for (int b = threadIdx.x; b < nb; b += blockDim.x) ww += part[b];
   ww = block_sum(ww, lds);
   arnoldi_column(L, j, sm, ww, est);
+}
+
+// ---------------------------------------------------------------- lagged CGS2
+// Arnoldi step j with the second (re)orthogonalisation of each basis vector
+// delayed by one step, so that the step makes two passes over the basis where
+// CGS2 makes three (and needs no separate normalisation). Entering step j:
+// v_0..v_{j-1} final and orthonormal, u = V[:, j] the previous step's vector
+// orthogonalised once (unnormalised), w = D^-1 A z_j with z_j = M u (the
+// V-cycle ran on u itself: FGMRES needs A z_j expressed in the orthonormal basis,
+// not z_j tied to a basis vector, and M is linear).
+//   dots    a = V^T u, c = V^T w, s = u.u, d = u.w                   (one pass)
+//   colA    nu = sqrt(s - a.a) (1 for j = 0: u = v_0 is final); column j-1 of
+//           H is final: hp_{j-1} + a, nu (hp: its projections at step j-1);
+//           its Givens rotation; provisional column j: c, e = (d - a.c)/nu
+//   update  v_j = (u - V a)/nu, w1 = w - V c - v_j e -> V[:, j+1], the next
+//           V-cycle input S .* w1, and ||w1||^2                       (one pass)
+//   colB    the residual estimate of step j from the provisional column
+//           (c, e, ||w1||) under the final rotations 0..j-1
+// v_j is exactly CGS2's: u reorthogonalised against the final basis; column j
+// of H is completed one step later (or by a closing dots + colA when the cycle
+// stops after step j).
+// P lanes share a row (P = 1 or 2), lane h of the pair taking columns t = h + P m:
+// the accumulators (and coefficients) per thread fall from 2K to 2K/P doubles.
+template <int P, class T>
+__device__ __forceinline__ T pair_sum(T v) {  // sum over the lanes of one column set (xor offsets >= P)
+#pragma unroll
+  for (int off = kWave / 2; off >= P; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+template <int K, int P>
+__global__ __launch_bounds__(kGsThreads) void k_lag_dots(int k, const double* __restrict__ V, long long n,
+                                                         const double* __restrict__ u, const double* __restrict__ w,
+                                                         double* __restrict__ part) {
+  constexpr int KP = K / P;
+  constexpr int NC = 2 * K + 2;
+  constexpr int NW = kGsThreads / kWave;
+  __shared__ double lds[NW * NC];
+  const int h = threadIdx.x % P;
+  double aa[KP], cc[KP];
+#pragma unroll
+  for (int m = 0; m < KP; ++m) aa[m] = cc[m] = 0.0;
+  double ss = 0.0, dd = 0.0;
+  const long long rows_per_pass = (long long)gridDim.x * blockDim.x / P;
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
+    const double ui = u[i], wi = w[i];
+#pragma unroll
+    for (int m = 0; m < KP; ++m) {
+      const int t = h + P * m;
+      const double v = (t < k) ? V[(long long)t * n + i] : 0.0;
+      aa[m] += v * ui;
+      cc[m] += v * wi;
+    }
+    if (h == 0) {
+      ss += ui * ui;
+      dd += ui * wi;
+    }
+  }
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int m = 0; m < KP; ++m) {
+    if (P * m < k) {
+      const double x = pair_sum<P>(aa[m]), y = pair_sum<P>(cc[m]);
+      const int t = lane + P * m;  // lanes 0..P-1 hold the sums of columns P m .. P m + P - 1
+      if (lane < P && t < k) {
+        lds[wid * NC + t] = x;
+        lds[wid * NC + K + t] = y;
+      }
+    }
+  }
+  {
+    const double x = wave_sum(ss), y = wave_sum(dd);
+    if (lane == 0) {
+      lds[wid * NC + 2 * K] = x;
+      lds[wid * NC + 2 * K + 1] = y;
+    }
+  }
+  __syncthreads();
+  // partial columns: a_t -> t, c_t -> k + t, s -> 2k, d -> 2k + 1
+  for (int q = threadIdx.x; q < 2 * k + 2; q += blockDim.x) {
+    const int src = (q < k) ? q : (q < 2 * k ? K + (q - k) : 2 * K + (q - 2 * k));
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < NW; ++r) v += lds[r * NC + src];
+    part[(long long)q * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+template <int K, int P, class TO>
+__global__ __launch_bounds__(kGsThreads) void k_lag_update(int k, double* __restrict__ V, long long n,
+                                                           const double* __restrict__ w,
+                                                           const double* __restrict__ ca,
+                                                           const double* __restrict__ cc,
+                                                           const double* __restrict__ lag,
+                                                           const double* __restrict__ S, TO* __restrict__ bout,
+                                                           double* __restrict__ part) {
+  constexpr int KP = K / P;
+  __shared__ double lds[kGsThreads / kWave];
+  const int h = threadIdx.x % P;
+  double a[KP], c[KP];
+#pragma unroll
+  for (int m = 0; m < KP; ++m) {
+    const int t = h + P * m;
+    a[m] = (t < k) ? ca[t] : 0.0;
+    c[m] = (t < k) ? cc[t] : 0.0;
+  }
+  const double inv_nu = lag[0], e = lag[1];
+  double* u = V + (long long)k * n;
+  double* un = u + n;
+  double acc = 0.0;
+  const long long rows_per_pass = (long long)gridDim.x * blockDim.x / P;
+  // every lane of a pair walks the same rows (the pair folds with one shuffle)
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
+    double su = 0.0, sw = 0.0;
+#pragma unroll
+    for (int m = 0; m < KP; ++m) {
+      const int t = h + P * m;
+      if (t < k) {
+        const double v = V[(long long)t * n + i];
+        su += v * a[m];
+        sw += v * c[m];
+      }
+    }
+    if (P > 1) {
+      su += __shfl_xor(su, 1, kWave);
+      sw += __shfl_xor(sw, 1, kWave);
+    }
+    if (h == 0) {
+      const double vj = (u[i] - su) * inv_nu;
+      const double w1 = (w[i] - sw) - vj * e;
+      u[i] = vj;
+      un[i] = w1;
+      bout[i] = (TO)(S[i] * w1);
+      acc += w1 * w1;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// sum of nb partials of column q, folded by one wavefront in a fixed order
+__device__ __forceinline__ double fold_col(const double* __restrict__ part, int nb, int q) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double v = 0.0;
+  for (int b = lane; b < nb; b += kWave) v += part[(long long)q * nb + b];
+  return wave_sum(v);
+}
+
+// c, s = the Givens rotation that zeroes b under a
+__device__ __forceinline__ void givens(double a, double b, double& c, double& s) {
+  if (b == 0.0) { c = 1.0; s = 0.0; }
+  else if (fabs(b) > fabs(a)) { const double t = a / b; s = 1.0 / sqrt(1.0 + t * t); c = s * t; }
+  else { const double t = b / a; c = 1.0 / sqrt(1.0 + t * t); s = c * t; }
+}
+
+// All 2j + 2 columns of partials fold at once: 8 lanes per column (128 column
+// slots in 1024 threads), each lane summing every 8th partial, then the 8 lanes.
+constexpr int kColAThreads = 1024;
+constexpr int kColALanes = 8;
+static_assert(kColAThreads / kColALanes >= 2 * kMaxFusedGs + 2, "one lane group per column");
+__global__ __launch_bounds__(kColAThreads) void k_lag_colA(Layout L, int j, double* __restrict__ sm,
+                                                           const double* __restrict__ part, int nb) {
+  __shared__ double val[2 * kMaxFusedGs + 2];
+  {
+    const int q = threadIdx.x / kColALanes, g = threadIdx.x % kColALanes;
+    double v = 0.0;
+    if (q < 2 * j + 2) {
+#pragma unroll 8
+      for (int b = g; b < nb; b += kColALanes) v += part[(long long)q * nb + b];
+    }
+    v = group_sum<kColALanes>(v);
+    if (q < 2 * j + 2 && g == 0) val[q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* a = val;
+  const double* c = val + j;
+  const double ss = val[2 * j], dd = val[2 * j + 1];
+  double aa = 0.0, ac = 0.0;
+  for (int t = 0; t < j; ++t) {
+    aa += a[t] * a[t];
+    ac += a[t] * c[t];
+  }
+  const double nu = (j == 0) ? 1.0 : sqrt(fmax(ss - aa, 0.0));
+  double* hp = sm + L.hp();
+  double* g = sm + L.g();
+  double* cs = sm + L.cs();
+  double* sn = sm + L.sn();
+  if (j > 0) {  // column j-1 is final: hp + a over rows 0..j-1, nu in row j
+    double col[kMaxFusedGs + 1];
+    for (int t = 0; t < j; ++t) col[t] = hp[t] + a[t];
+    col[j] = nu;
+    for (int t = 0; t + 1 < j; ++t) {
+      const double x = col[t], y = col[t + 1];
+      col[t] = cs[t] * x + sn[t] * y;
+      col[t + 1] = -sn[t] * x + cs[t] * y;
+    }
+    double cr, sr;
+    givens(col[j - 1], col[j], cr, sr);
+    cs[j - 1] = cr;
+    sn[j - 1] = sr;
+    col[j - 1] = cr * col[j - 1] + sr * col[j];
+    col[j] = 0.0;
+    for (int t = 0; t <= j; ++t) sm[L.H(t, j - 1)] = col[t];
+    g[j] = -sr * g[j - 1];
+    g[j - 1] = cr * g[j - 1];
+  }
+  const double e = (nu > 0.0) ? (dd - ac) / nu : 0.0;
+  for (int t = 0; t < j; ++t) {
+    hp[t] = c[t];
+    sm[L.h() + t] = a[t];
+    sm[L.h2() + t] = c[t];
+  }
+  hp[j] = e;
+  sm[L.lag()] = (nu > 0.0) ? 1.0 / nu : 0.0;
+  sm[L.lag() + 1] = e;
+}
+
+// residual estimate of step j from its provisional column (hp, ||w1||): stored in
+// sc[2] and in *est (pinned host memory)
+__global__ __launch_bounds__(kGsThreads) void k_lag_colB(Layout L, int j, double* __restrict__ sm,
+                                                         const double* __restrict__ part, int nb, double* est) {
+  __shared__ double ww;
+  if (threadIdx.x < kWave) {
+    const double v = fold_col(part, nb, 0);
+    if (threadIdx.x == 0) ww = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* hp = sm + L.hp();
+  const double* cs = sm + L.cs();
+  const double* sn = sm + L.sn();
+  double x = hp[0];
+  for (int t = 0; t < j; ++t) {  // rotations 0..j-1 applied to the column; only its last two entries matter
+    const double y = hp[t + 1];
+    x = -sn[t] * x + cs[t] * y;
+  }
+  double cr, sr;
+  givens(x, sqrt(fmax(ww, 0.0)), cr, sr);
+  double* sc = sm + L.sc();
+  sc[2] = fabs(sr * sm[L.g() + j]);
+  *est = sc[2];
+}
+
+template <int K>
+void launch_lag_pair(int k, double* V, long long n, const double* w, double* sm, Layout L, const double* S, void* bout,
+                     int precision, double* part, bool dots_only, const double* u, hipStream_t s) {
+  constexpr int P = K >= 16 ? 2 : 1;  // two lanes per row where the accumulators would crowd the registers
+  if (dots_only) {
+    k_lag_dots<K, P><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, u, w, part);
+  } else if (precision == 32) {
+    k_lag_update<K, P, float><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(), S,
+                                                               (float*)bout, part);
+  } else {
+    k_lag_update<K, P, double><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(),
+                                                                S, (double*)bout, part);
+  }
+  IMPES_LAUNCH_CHECK();
+}
+
+// the dots pass (dots_only) or the update pass of step k, by the basis size k
+void launch_lag(int k, double* V, long long n, const double* w, double* sm, Layout L, const double* S, void* bout,
+                int precision, double* part, bool dots_only, const double* u, hipStream_t s) {
+  if (k <= 4) launch_lag_pair<4>(k, V, n, w, sm, L, S, bout, precision, part, dots_only, u, s);
+  else if (k <= 8) launch_lag_pair<8>(k, V, n, w, sm, L, S, bout, precision, part, dots_only, u, s);
+  else if (k <= 16) launch_lag_pair<16>(k, V, n, w, sm, L, S, bout, precision, part, dots_only, u, s);
+  else launch_lag_pair<kMaxFusedGs>(k, V, n, w, sm, L, S, bout, precision, part, dots_only, u, s);
 }
 
 // v = w / nrm[0] and the next V-cycle input b = a .* v (in the hierarchy's
@@ -217,6 +495,8 @@ Fgmres::Fgmres() {
   const char* e = std::getenv("IMPES_FGMRES_LOOKAHEAD");
   lookahead_ = e && e[0] == '1';
   no_defer_ = e && e[0] == '0';
+  const char* g = std::getenv("IMPES_FGMRES_CGS2");
+  cgs2_ = g && g[0] == '1';
 }
 
 Fgmres::~Fgmres() {
@@ -241,7 +521,8 @@ void Fgmres::ensure(long long n, int m) {
   bs_.alloc(n);
   Layout L{m};
   small_.alloc(L.size());
-  work_.alloc((size_t)std::max(kRedBlocks, kMultiDotBlocks) * (m + 1));
+  work_.alloc(std::max({(size_t)kRedBlocks * (m + 1), (size_t)kMultiDotBlocks * (m + 1),
+                        (size_t)kGsBlocks * (2 * m + 2)}));
 }
 
 void Fgmres::apply(const Operator& A, const double* x, double* y, hipStream_t s) {
@@ -332,6 +613,10 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
   const double ratio = std::max(err_ratio_[0], err_ratio_[1]);
   if (check && ratio > 0.0) target = std::min(tol, kErrTarget * etol_ / ratio * bnorm);
   bool ok = false;
+  // two passes over the basis per Arnoldi step (lagged reorthogonalisation) when
+  // a preconditioner stages the next V-cycle input and the basis fits the fused
+  // kernels; else CGS2's three passes and a normalisation
+  const bool lagged = M != nullptr && m <= kMaxFusedGs && !cgs2_;
   for (;;) {
     while (beta > target && its < maxit && std::isfinite(beta)) {
       k_init_cycle<<<1, 1, 0, s>>>(L, sm);
@@ -350,20 +635,35 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
       // last two residual estimates read (the cycle's initial residual first)
       double e1 = beta, e2 = -1.0;
       int pending = -1;  // iteration whose estimate is still to be read
+      // lagged orthogonalisation: columns of H final so far (the lagged loop only)
+      int final_cols = 0;
       for (int j = 0; j < m; ++j) {
         double* vj = V + (long long)j * n;
         double* zj = Z + (long long)j * n;
         double* est = host_dev_ + 2 + (j & 1);
+        if (lagged) {
+          // z_j = M u (the V-cycle input S .* u was written by the previous update,
+          // or by the cycle's normalisation for j = 0), w = D^-1 A z_j
+          precondition_apply(vj, zj);
+          launch_lag(j, V, n, w_.p, sm, L, nullptr, nullptr, precision, work_.p, true, vj, s);
+          k_lag_colA<<<1, kColAThreads, 0, s>>>(L, j, sm, work_.p, kGsBlocks);
+          IMPES_LAUNCH_CHECK();
+          final_cols = j;
+          launch_lag(j, V, n, w_.p, sm, L, M->in_scale(), M->vcycle_in(precision), precision, work_.p, false,
+                     nullptr, s);
+          k_lag_colB<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kGsBlocks, est);
+          IMPES_LAUNCH_CHECK();
+        } else {
         precondition_apply(vj, zj);
         // CGS2: h = V^T w; then w -= V h with h2 = V^T w in one sweep; then w -= V h2
         // with ||w||^2 and the rotations
         const int kj = j + 1;
         if (kj <= kMaxFusedGs) {
           launch_gs_pass<0>(kj, w_.p, V, n, nullptr, work_.p, s);
-          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h());
+          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kGsBlocks, sm + L.h());
           IMPES_LAUNCH_CHECK();
           launch_gs_pass<1>(kj, w_.p, V, n, sm + L.h(), work_.p, s);
-          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kMultiDotBlocks, sm + L.h2());
+          k_fold_cols<<<kj, kGsThreads, 0, s>>>(work_.p, kGsBlocks, sm + L.h2());
           IMPES_LAUNCH_CHECK();
           launch_gs_pass<2>(kj, w_.p, V, n, sm + L.h2(), work_.p, s);
         } else {
@@ -373,9 +673,10 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
           launch_gs_update_f64(w_.p, V, n, kj, sm + L.h2(), n, s);
           launch_dot_partial_f64(w_.p, w_.p, n, work_.p, kMultiDotBlocks, s);
         }
-        k_arnoldi<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kMultiDotBlocks, est);
+        k_arnoldi<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kj <= kMaxFusedGs ? kGsBlocks : kMultiDotBlocks, est);
         IMPES_LAUNCH_CHECK();
         normalize(V + (long long)(j + 1) * n, w_.p, sc + 3);
+        }
         post(j);
         if (pending >= 0) {  // iteration j - 1 was not tested before j was enqueued
           const double e = read_est(pending);
@@ -404,6 +705,13 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
         if (stops(j, e)) break;
       }
       its = its0 + k;
+      if (lagged && final_cols < k) {
+        // close column k-1: dots of u = V[:, k] against v_0..v_{k-1}, then its colA
+        const double* uk = V + (long long)k * n;
+        launch_lag(k, V, n, uk, sm, L, nullptr, nullptr, precision, work_.p, true, uk, s);
+        k_lag_colA<<<1, kColAThreads, 0, s>>>(L, k, sm, work_.p, kGsBlocks);
+        IMPES_LAUNCH_CHECK();
+      }
       k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
       IMPES_LAUNCH_CHECK();
       launch_lincomb_add_f64(x, Z, n, k, sm + L.y(), n, s);

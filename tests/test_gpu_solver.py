@@ -452,6 +452,38 @@ def test_deferred_convergence_tests_change_nothing(dev, restart, monkeypatch):
         assert out[mode][1:] == out[None][1:], (mode, out[mode][1:], out[None][1:])
 
 
+@pytest.mark.parametrize("restart,precision", [(30, 32), (6, 32), (30, 64), (32, 32)])
+def test_lagged_orthogonalisation_matches_cgs2(dev, restart, precision, monkeypatch):
+    """The default Arnoldi step (reorthogonalisation lagged by one step: two
+    passes over the basis) against CGS2's three passes (IMPES_FGMRES_CGS2=1):
+    the same basis up to rounding, so the same iteration and restart counts and
+    solutions within the solve tolerance of each other and of scipy's direct
+    solve; also at the largest restart of the lagged form (32) and across
+    restarts (6)."""
+    import impes_hip as H
+
+    dims = (24, 20, 8)
+    A, d, ox, oy, oz = impes_like(*dims, seed=12)
+    xt = 2e7 + 1e5 * np.random.default_rng(6).standard_normal(A.shape[0])
+    b = A @ xt
+    hg = H.AmgHierarchy()
+    hg.setup_st7(dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev))
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("IMPES_FGMRES_CGS2", mode)
+        solver = H.FgmresSolver()  # reads the switch when created
+        x = torch.zeros(A.shape[0], dtype=torch.float64, device=dev)
+        st = solver.solve_st7(hg, dims, t(d, dev), t(ox, dev), t(oy, dev), t(oz, dev), t(1.0 / d, dev), t(b, dev),
+                              x, precision=precision, rtol=1e-10, restart=restart, maxit=300)
+        assert st["converged"], (mode, st)
+        out[mode] = (x.cpu().numpy(), st["iterations"], st["restarts"])
+    assert out["0"][1:] == out["1"][1:], (out["0"][1:], out["1"][1:])
+    xd = spl.spsolve(sp.csc_matrix(A), b)
+    for mode in ("0", "1"):
+        assert rel(out[mode][0], xd) < 1e-8, mode
+    assert rel(out["0"][0], out["1"][0]) < 1e-8
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 @pytest.mark.parametrize("dims", [(60, 60, 1), (20, 20, 10)])
 def test_fgmres_matches_direct_solve(dev, precision, dims):
