@@ -343,12 +343,8 @@ static int vcycle_entry(const char* fn, impes_amg* amg, int precision, const dou
       launch_f64_to_f32(b32.p, b, n, s);
       if (!x_is_zero) launch_f64_to_f32(x32.p, x, n, s);
       amg->h.vcycle32(b32.p, x32.p, x_is_zero != 0, s);
-      // widen back
-      DevBuf<double> ones;
-      ones.alloc(n);
-      launch_fill_f64(ones.p, 1.0, n, s);
-      launch_scale_f32_to_f64(x, ones.p, x32.p, n, s);
-      IMPES_HIP_CHECK(hipStreamSynchronize(s));
+      launch_f32_to_f64(x, x32.p, n, s);
+      IMPES_HIP_CHECK(hipStreamSynchronize(s));  // b32 and x32 are released on return
     }
   });
 }

@@ -554,20 +554,26 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
   };
   st = SolveStats{};
   st.precision = precision;
-  // ||D^-1 b||
+  double* V = V_.p;
+  double* Z = Z_.p;
+  // ||D^-1 b|| and the scaled initial residual r0 = D^-1 (b - A x0) in V, their
+  // squared norms read back with one synchronisation
   launch_scale_f64(bs_.p, A.dinv, b, n, s);
   launch_dot_f64(bs_.p, bs_.p, n, sc + 5, work_.p, s);
-  const double bnorm = std::sqrt(read(sc + 5));
-  st.bnorm = bnorm;
   if (!use_x0) launch_fill_f64(x, 0.0, n, s);
+  residual(A, x, b, V, s);
+  launch_dot_f64(V, V, n, sc + 4, work_.p, s);
+  IMPES_HIP_CHECK(hipMemcpyAsync(host_ + 4, sc + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(host_[5]);
+  double beta = std::sqrt(host_[4]);
+  st.bnorm = bnorm;
   if (bnorm == 0.0) {
     launch_fill_f64(x, 0.0, n, s);
     st.converged = 1;
     return;
   }
   const double tol = rtol * bnorm;
-  double* V = V_.p;
-  double* Z = Z_.p;
   // v = w / nrm, and with a preconditioner also its scaled V-cycle input
   auto normalize = [&](double* v, const double* w, const double* nrm) {
     if (!M) {
@@ -601,9 +607,6 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
     else launch_scale_f64(zj, M->out_scale(), (const double*)M->vcycle_out(64), n, s);
     apply(A, zj, w_.p, s);
   };
-  residual(A, x, b, V, s);
-  launch_dot_f64(V, V, n, sc + 4, work_.p, s);
-  double beta = std::sqrt(read(sc + 4));
   st.r0_rel = beta / bnorm;
   int its = 0;
   const bool check = etol_ > 0.0 && M != nullptr;
