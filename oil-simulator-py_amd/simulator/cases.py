@@ -29,9 +29,10 @@ SEEDS = {"field": 0, "medians": 1, "truth": 0, "prior": 1, "noise": 2, "fd": 3}
 # start, stopped at a 1e-8 relative residual AND an estimated relative pressure
 # error below 1e-8 (one preconditioner application per check); coarsening stops
 # at 600 rows, where the coarsest level (508 rows at 1M cells) is inverted densely
-# on the matrix cores at every update (3% faster per step than the 102-row
-# coarsest level of the default 200/500). The GPU parity tests run the shrunk
-# benchmark cases with exactly these settings.
+# on the matrix cores at every update (5-7% faster per step than the 102-row
+# coarsest level of the default 200/500; a 1254-row coarsest level runs about as
+# fast). The GPU parity tests run the shrunk benchmark cases with exactly these
+# settings.
 BENCH_AMG = {"tol": 1e-8, "error_tol": 1e-8, "precision": "fp32", "restart": 30, "initial_guess": "extrapolate",
              "max_levels": 25, "max_iter": 400, "coarsest_size": 600, "dense_max": 600}
 
