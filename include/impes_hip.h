@@ -275,7 +275,8 @@ int impes_model_assemble(impes_model* m, double dt, const double* p, const doubl
 int impes_model_solve_pressure(impes_model* m, int precision, double rtol, int restart, int maxit,
                                const double* p_prev, double* p_new, impes_solve_stats* stats, void* stream);
 /* The pressure-error check of impes_solver_set_error_tol for this model's solves
- * (0 = off, the default); the north star's "pressure to 1e-8 relative" bar. */
+ * (0 = off, the default): holds the pressure to etol relative to the exact solve
+ * where the residual stop alone would not. No counterpart in the reference. */
 int impes_model_set_error_tol(impes_model* m, double etol);
 /* Jacobi-preconditioned CG, reference semantics (absolute stop on sqrt(r.z), x0 = 0):
  * Simulator._solve_pressure_cg_pytorch, simulation.py:1662-1697. converged/iterations: host. */
