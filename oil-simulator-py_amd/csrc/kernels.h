@@ -66,6 +66,9 @@ struct Csr4F32 {
   const int* rp4 = nullptr;
   const int4* ci4 = nullptr;
   const float4* v4 = nullptr;
+  // when set (every |column - row| < 2^15): 16-bit column offsets from the row,
+  // read instead of ci4 (8 bytes per 4 entries instead of 16)
+  const short4* co4 = nullptr;
 };
 // lanes per row of the padded kernels from the padded entry count
 int csr4_group_for(int n_rows, long long nnz4);
@@ -74,6 +77,9 @@ void launch_csr4_count(int n, const int* rp, int* cnt, hipStream_t s);
 // padded pattern (ci4 sized rp4[n]) and FP32 values (rounded from FP64) of a CSR matrix
 void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s);
 void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s);
+// co4[e] = ci4[e] - row (16-bit) over nnz4 entries; *maxabs = the largest |ci4[e] - row|
+// (a device int, zero on entry)
+void launch_csr4_offsets(int n, const int* rp4, const int* ci4, short* co4, int* maxabs, hipStream_t s);
 // r = b - A x and xo = xi + w inv_relax .* (b - A xi) on the padded form
 void launch_residual4_f32(const Csr4F32& A, int group, const float* x, const float* b, float* r, hipStream_t s);
 void launch_jacobi4_f32(const Csr4F32& A, int group, const float* xi, const float* b, const float* inv_relax, float w,

@@ -69,11 +69,11 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
 }
 
 // Padded FP32 CSR (Csr4F32): G lanes per row, each lane 4 entries per 16-byte load.
-template <int G, int MODE>
+template <int G, int MODE, bool OFF16>
 __global__ __launch_bounds__(kBlock) void k_csr4(int n, const int* __restrict__ rp4, const int4* __restrict__ ci4,
-                                                  const float4* __restrict__ v4, const float* __restrict__ x,
-                                                  const float* __restrict__ b, const float* __restrict__ aux, float w,
-                                                  float* __restrict__ y) {
+                                                  const short4* __restrict__ co4, const float4* __restrict__ v4,
+                                                  const float* __restrict__ x, const float* __restrict__ b,
+                                                  const float* __restrict__ aux, float w, float* __restrict__ y) {
   constexpr int RPB = kBlock / G;
   const int g = threadIdx.x % G;
   for (long long base = xcd_block() * RPB; base < n; base += (long long)gridDim.x * RPB) {
@@ -94,7 +94,14 @@ __global__ __launch_bounds__(kBlock) void k_csr4(int n, const int* __restrict__ 
 #pragma unroll 2
       for (int q = (rp4[row] >> 2) + g; q < q1; q += G) {
         const float4 a = v4[q];
-        const int4 c = ci4[q];
+        int4 c;
+        if (OFF16) {
+          const short4 o = co4[q];
+          const int r = (int)row;
+          c = make_int4(r + o.x, r + o.y, r + o.z, r + o.w);
+        } else {
+          c = ci4[q];
+        }
         sum += a.x * x[c.x];
         sum += a.y * x[c.y];
         sum += a.z * x[c.z];
@@ -109,22 +116,42 @@ __global__ __launch_bounds__(kBlock) void k_csr4(int n, const int* __restrict__ 
   }
 }
 
+template <int MODE, bool OFF16>
+void csr4_launch(const Csr4F32& A, int G, const float* x, const float* b, const float* aux, float w, float* y,
+                 hipStream_t s) {
+  const int grid = grid_for(A.n, kBlock / G, 1 << 18);
+#define IMPES_CSR4_CASE(GG)                                                                                     \
+  case GG:                                                                                                     \
+    klaunch(k_csr4<GG, MODE, OFF16>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.co4, A.v4, x, b, aux, \
+            w, y);                                                                                             \
+    break;
+  switch (G) {
+    IMPES_CSR4_CASE(1)
+    IMPES_CSR4_CASE(2)
+    IMPES_CSR4_CASE(4)
+    IMPES_CSR4_CASE(8)
+    IMPES_CSR4_CASE(16)
+    IMPES_CSR4_CASE(32)
+    default:
+      klaunch(k_csr4<64, MODE, OFF16>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.co4, A.v4, x, b, aux,
+              w, y);
+      break;
+  }
+#undef IMPES_CSR4_CASE
+  IMPES_LAUNCH_CHECK();
+}
+
 template <int MODE>
 void csr4_dispatch(const Csr4F32& A, int G, const float* x, const float* b, const float* aux, float w, float* y,
                    hipStream_t s) {
   if (A.n <= 0) return;
   IMPES_REQUIRE(((uintptr_t)A.ci4 & 15) == 0 && ((uintptr_t)A.v4 & 15) == 0, "padded CSR: arrays not 16-byte aligned");
-  const int grid = grid_for(A.n, kBlock / G, 1 << 18);
-  switch (G) {
-    case 1: klaunch(k_csr4<1, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    case 2: klaunch(k_csr4<2, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    case 4: klaunch(k_csr4<4, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    case 8: klaunch(k_csr4<8, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    case 16: klaunch(k_csr4<16, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    case 32: klaunch(k_csr4<32, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
-    default: klaunch(k_csr4<64, MODE>, dim3(grid), dim3(kBlock), 0, s, A.n, A.rp4, A.ci4, A.v4, x, b, aux, w, y); break;
+  if (A.co4) {
+    IMPES_REQUIRE(((uintptr_t)A.co4 & 7) == 0, "padded CSR: offsets not 8-byte aligned");
+    csr4_launch<MODE, true>(A, G, x, b, aux, w, y, s);
+  } else {
+    csr4_launch<MODE, false>(A, G, x, b, aux, w, y, s);
   }
-  IMPES_LAUNCH_CHECK();
 }
 
 __global__ void k_csr4_count(int n, const int* __restrict__ rp, int* __restrict__ cnt) {
@@ -138,6 +165,20 @@ __global__ void k_csr4_pattern(int n, const int* __restrict__ rp, const int* __r
     const int e0 = rp[r], len = rp[r + 1] - e0, q0 = rp4[r], len4 = rp4[r + 1] - q0;
     for (int k = 0; k < len4; ++k) ci4[q0 + k] = (k < len) ? ci[e0 + k] : (int)r;
   }
+}
+
+__global__ void k_csr4_offsets(int n, const int* __restrict__ rp4, const int* __restrict__ ci4,
+                               short* __restrict__ co4, int* __restrict__ maxabs) {
+  int m = 0;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
+    for (int e = rp4[r]; e < rp4[r + 1]; ++e) {
+      const int d = ci4[e] - (int)r;
+      m = max(m, abs(d));
+      co4[e] = (short)max(-32768, min(32767, d));
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & (kWave - 1)) == 0 && m > 0) atomicMax(maxabs, m);
 }
 
 // G lanes per row (entries strided by G): consecutive lanes write consecutive entries
@@ -507,6 +548,11 @@ void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, in
 void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s) {
   if (n <= 0) return;
   k_csr4_values<8><<<grid_for((long long)n * 8, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, v, rp4, v4);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_csr4_offsets(int n, const int* rp4, const int* ci4, short* co4, int* maxabs, hipStream_t s) {
+  if (n <= 0) return;
+  k_csr4_offsets<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp4, ci4, co4, maxabs);
   IMPES_LAUNCH_CHECK();
 }
 void launch_residual4_f32(const Csr4F32& A, int G, const float* x, const float* b, float* r, hipStream_t s) {
