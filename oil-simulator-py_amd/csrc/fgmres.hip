@@ -332,14 +332,6 @@ __global__ __launch_bounds__(kGsThreads) void k_lag_update(int k, double* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// sum of nb partials of column q, folded by one wavefront in a fixed order
-__device__ __forceinline__ double fold_col(const double* __restrict__ part, int nb, int q) {
-  const int lane = threadIdx.x & (kWave - 1);
-  double v = 0.0;
-  for (int b = lane; b < nb; b += kWave) v += part[(long long)q * nb + b];
-  return wave_sum(v);
-}
-
 // c, s = the Givens rotation that zeroes b under a
 __device__ __forceinline__ void givens(double a, double b, double& c, double& s) {
   if (b == 0.0) { c = 1.0; s = 0.0; }
@@ -347,24 +339,28 @@ __device__ __forceinline__ void givens(double a, double b, double& c, double& s)
   else { const double t = b / a; c = 1.0 / sqrt(1.0 + t * t); s = c * t; }
 }
 
-// All 2j + 2 columns of partials fold at once: 8 lanes per column (128 column
-// slots in 1024 threads), each lane summing every 8th partial, then the 8 lanes.
+// All 2j + 2 columns of partials fold at once: G lanes per column (G = 16 while
+// the columns fit 64 slots of 1024 threads, else 8), each lane summing every G-th
+// partial, then the G lanes.
 constexpr int kColAThreads = 1024;
-constexpr int kColALanes = 8;
-static_assert(kColAThreads / kColALanes >= 2 * kMaxFusedGs + 2, "one lane group per column");
+static_assert(kColAThreads / 8 >= 2 * kMaxFusedGs + 2, "one lane group per column");
+template <int G>
+__device__ __forceinline__ void fold_cols(const double* __restrict__ part, int nb, int ncols, double* val) {
+  const int q = threadIdx.x / G, g = threadIdx.x % G;
+  double v = 0.0;
+  if (q < ncols) {
+#pragma unroll 8
+    for (int b = g; b < nb; b += G) v += part[(long long)q * nb + b];
+  }
+  v = group_sum<G>(v);
+  if (q < ncols && g == 0) val[q] = v;
+}
+
 __global__ __launch_bounds__(kColAThreads) void k_lag_colA(Layout L, int j, double* __restrict__ sm,
                                                            const double* __restrict__ part, int nb) {
   __shared__ double val[2 * kMaxFusedGs + 2];
-  {
-    const int q = threadIdx.x / kColALanes, g = threadIdx.x % kColALanes;
-    double v = 0.0;
-    if (q < 2 * j + 2) {
-#pragma unroll 8
-      for (int b = g; b < nb; b += kColALanes) v += part[(long long)q * nb + b];
-    }
-    v = group_sum<kColALanes>(v);
-    if (q < 2 * j + 2 && g == 0) val[q] = v;
-  }
+  if (2 * j + 2 <= kColAThreads / 16) fold_cols<16>(part, nb, 2 * j + 2, val);
+  else fold_cols<8>(part, nb, 2 * j + 2, val);
   __syncthreads();
   if (threadIdx.x != 0) return;
   const double* a = val;
@@ -412,14 +408,12 @@ __global__ __launch_bounds__(kColAThreads) void k_lag_colA(Layout L, int j, doub
 
 // residual estimate of step j from its provisional column (hp, ||w1||): stored in
 // sc[2] and in *est (pinned host memory)
-__global__ __launch_bounds__(kGsThreads) void k_lag_colB(Layout L, int j, double* __restrict__ sm,
-                                                         const double* __restrict__ part, int nb, double* est) {
-  __shared__ double ww;
-  if (threadIdx.x < kWave) {
-    const double v = fold_col(part, nb, 0);
-    if (threadIdx.x == 0) ww = v;
-  }
-  __syncthreads();
+__global__ __launch_bounds__(kColAThreads) void k_lag_colB(Layout L, int j, double* __restrict__ sm,
+                                                           const double* __restrict__ part, int nb, double* est) {
+  __shared__ double lds[kColAThreads / kWave];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+  const double ww = block_sum(v, lds);
   if (threadIdx.x != 0) return;
   const double* hp = sm + L.hp();
   const double* cs = sm + L.cs();
@@ -654,7 +648,7 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
           final_cols = j;
           launch_lag(j, V, n, w_.p, sm, L, M->in_scale(), M->vcycle_in(precision), precision, work_.p, false,
                      nullptr, s);
-          k_lag_colB<<<1, kGsThreads, 0, s>>>(L, j, sm, work_.p, kGsBlocks, est);
+          k_lag_colB<<<1, kColAThreads, 0, s>>>(L, j, sm, work_.p, kGsBlocks, est);
           IMPES_LAUNCH_CHECK();
         } else {
         precondition_apply(vj, zj);
