@@ -339,9 +339,10 @@ __device__ __forceinline__ void givens(double a, double b, double& c, double& s)
   else { const double t = b / a; c = 1.0 / sqrt(1.0 + t * t); s = c * t; }
 }
 
-// All 2j + 2 columns of partials fold at once: G lanes per column (G = 16 while
-// the columns fit 64 slots of 1024 threads, else 8), each lane summing every G-th
-// partial, then the G lanes.
+// All 2j + 2 columns of partials fold at once: G lanes per column (the widest
+// power of two up to a wave with G * columns <= 1024 threads), each lane summing
+// every G-th partial, then the G lanes. The partials were written by other XCDs,
+// so each load is a far round trip: a lane keeps 16 of them in flight.
 constexpr int kColAThreads = 1024;
 static_assert(kColAThreads / 8 >= 2 * kMaxFusedGs + 2, "one lane group per column");
 template <int G>
@@ -349,7 +350,7 @@ __device__ __forceinline__ void fold_cols(const double* __restrict__ part, int n
   const int q = threadIdx.x / G, g = threadIdx.x % G;
   double v = 0.0;
   if (q < ncols) {
-#pragma unroll 8
+#pragma unroll 16
     for (int b = g; b < nb; b += G) v += part[(long long)q * nb + b];
   }
   v = group_sum<G>(v);
@@ -359,48 +360,67 @@ __device__ __forceinline__ void fold_cols(const double* __restrict__ part, int n
 __global__ __launch_bounds__(kColAThreads) void k_lag_colA(Layout L, int j, double* __restrict__ sm,
                                                            const double* __restrict__ part, int nb) {
   __shared__ double val[2 * kMaxFusedGs + 2];
-  if (2 * j + 2 <= kColAThreads / 16) fold_cols<16>(part, nb, 2 * j + 2, val);
-  else fold_cols<8>(part, nb, 2 * j + 2, val);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double* a = val;
-  const double* c = val + j;
-  const double ss = val[2 * j], dd = val[2 * j + 1];
-  double aa = 0.0, ac = 0.0;
-  for (int t = 0; t < j; ++t) {
-    aa += a[t] * a[t];
-    ac += a[t] * c[t];
-  }
-  const double nu = (j == 0) ? 1.0 : sqrt(fmax(ss - aa, 0.0));
+  __shared__ double rot[3 * kMaxFusedGs];  // hp, cs, sn of rows 0..j-1 as they were on entry
+  const int t = threadIdx.x;
   double* hp = sm + L.hp();
   double* g = sm + L.g();
   double* cs = sm + L.cs();
   double* sn = sm + L.sn();
-  if (j > 0) {  // column j-1 is final: hp + a over rows 0..j-1, nu in row j
-    double col[kMaxFusedGs + 1];
-    for (int t = 0; t < j; ++t) col[t] = hp[t] + a[t];
-    col[j] = nu;
-    for (int t = 0; t + 1 < j; ++t) {
-      const double x = col[t], y = col[t + 1];
-      col[t] = cs[t] * x + sn[t] * y;
-      col[t + 1] = -sn[t] * x + cs[t] * y;
-    }
-    double cr, sr;
-    givens(col[j - 1], col[j], cr, sr);
-    cs[j - 1] = cr;
-    sn[j - 1] = sr;
-    col[j - 1] = cr * col[j - 1] + sr * col[j];
-    col[j] = 0.0;
-    for (int t = 0; t <= j; ++t) sm[L.H(t, j - 1)] = col[t];
-    g[j] = -sr * g[j - 1];
-    g[j - 1] = cr * g[j - 1];
+  // the serial rotation below must not wait on far loads: fetch its rows now,
+  // while the fold's loads are in flight
+  double rh = 0.0, rc = 0.0, rs = 0.0;
+  if (t < j) {
+    rh = hp[t];
+    rc = cs[t];
+    rs = sn[t];
   }
-  const double e = (nu > 0.0) ? (dd - ac) / nu : 0.0;
-  for (int t = 0; t < j; ++t) {
+  const double gj1 = (j > 0) ? g[j - 1] : 0.0;
+  const int nc = 2 * j + 2;
+  if (nc <= kColAThreads / 64) fold_cols<64>(part, nb, nc, val);
+  else if (nc <= kColAThreads / 32) fold_cols<32>(part, nb, nc, val);
+  else if (nc <= kColAThreads / 16) fold_cols<16>(part, nb, nc, val);
+  else fold_cols<8>(part, nb, nc, val);
+  if (t < j) {
+    rot[t] = rh;
+    rot[kMaxFusedGs + t] = rc;
+    rot[2 * kMaxFusedGs + t] = rs;
+  }
+  __syncthreads();
+  const double* a = val;
+  const double* c = val + j;
+  if (t < j) {  // the next provisional column starts from c; a and c are kept for the update pass
     hp[t] = c[t];
     sm[L.h() + t] = a[t];
     sm[L.h2() + t] = c[t];
   }
+  if (t != 0) return;
+  const double ss = val[2 * j], dd = val[2 * j + 1];
+  double aa = 0.0, ac = 0.0;
+  for (int q = 0; q < j; ++q) {
+    aa += a[q] * a[q];
+    ac += a[q] * c[q];
+  }
+  const double nu = (j == 0) ? 1.0 : sqrt(fmax(ss - aa, 0.0));
+  if (j > 0) {  // column j-1 is final: hp + a over rows 0..j-1, nu in row j; rotations 0..j-2, then a new one
+    const double* oh = rot;
+    const double* oc = rot + kMaxFusedGs;
+    const double* os = rot + 2 * kMaxFusedGs;
+    double x = oh[0] + a[0];
+    for (int q = 0; q + 1 < j; ++q) {
+      const double y = oh[q + 1] + a[q + 1];
+      sm[L.H(q, j - 1)] = oc[q] * x + os[q] * y;
+      x = -os[q] * x + oc[q] * y;
+    }
+    double cr, sr;
+    givens(x, nu, cr, sr);
+    cs[j - 1] = cr;
+    sn[j - 1] = sr;
+    sm[L.H(j - 1, j - 1)] = cr * x + sr * nu;
+    sm[L.H(j, j - 1)] = 0.0;
+    g[j] = -sr * gj1;
+    g[j - 1] = cr * gj1;
+  }
+  const double e = (nu > 0.0) ? (dd - ac) / nu : 0.0;
   hp[j] = e;
   sm[L.lag()] = (nu > 0.0) ? 1.0 / nu : 0.0;
   sm[L.lag() + 1] = e;
@@ -411,22 +431,38 @@ __global__ __launch_bounds__(kColAThreads) void k_lag_colA(Layout L, int j, doub
 __global__ __launch_bounds__(kColAThreads) void k_lag_colB(Layout L, int j, double* __restrict__ sm,
                                                            const double* __restrict__ part, int nb, double* est) {
   __shared__ double lds[kColAThreads / kWave];
-  double v = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
-  const double ww = block_sum(v, lds);
-  if (threadIdx.x != 0) return;
+  __shared__ double rot[3 * kMaxFusedGs + 1];  // hp rows 0..j, cs and sn rows 0..j-1
+  const int t = threadIdx.x;
   const double* hp = sm + L.hp();
   const double* cs = sm + L.cs();
   const double* sn = sm + L.sn();
-  double x = hp[0];
-  for (int t = 0; t < j; ++t) {  // rotations 0..j-1 applied to the column; only its last two entries matter
-    const double y = hp[t + 1];
-    x = -sn[t] * x + cs[t] * y;
+  double rh = 0.0, rc = 0.0, rs = 0.0;
+  if (t <= j) rh = hp[t];
+  if (t < j) {
+    rc = cs[t];
+    rs = sn[t];
+  }
+  const double gj = sm[L.g() + j];
+  double v = 0.0;
+  for (int b = t; b < nb; b += blockDim.x) v += part[b];
+  if (t <= j) rot[t] = rh;
+  if (t < j) {
+    rot[kMaxFusedGs + 1 + t] = rc;
+    rot[2 * kMaxFusedGs + 1 + t] = rs;
+  }
+  const double ww = block_sum(v, lds);  // its barrier also publishes rot
+  if (t != 0) return;
+  const double* oc = rot + kMaxFusedGs + 1;
+  const double* os = rot + 2 * kMaxFusedGs + 1;
+  double x = rot[0];
+  for (int q = 0; q < j; ++q) {  // rotations 0..j-1 applied to the column; only its last two entries matter
+    const double y = rot[q + 1];
+    x = -os[q] * x + oc[q] * y;
   }
   double cr, sr;
   givens(x, sqrt(fmax(ww, 0.0)), cr, sr);
   double* sc = sm + L.sc();
-  sc[2] = fabs(sr * sm[L.g() + j]);
+  sc[2] = fabs(sr * gj);
   *est = sc[2];
 }
 
