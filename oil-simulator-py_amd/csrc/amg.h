@@ -101,7 +101,7 @@ struct Level {
   DevBuf<double> aug;
   // coarsest level above 160 rows: A_pinned^-1 padded to dense_pad(n) (matrix cores)
   bool mfma = false;
-  DevBuf<double> inv, gj_f, gj_p;
+  DevBuf<double> inv, gj_w, gj_p;
   DevBuf<int> gj_bad;
 };
 

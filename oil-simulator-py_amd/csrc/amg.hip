@@ -645,7 +645,7 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
   if (L.mfma) {
     const long long np = dense_pad(L.n);
     L.inv.alloc(np * np);
-    L.gj_f.alloc(np * 64);
+    L.gj_w.alloc(np * np);
     L.gj_p.alloc(64 * 64);
     L.gj_bad.alloc(1);
     dense_inverse(L, s);
@@ -655,7 +655,7 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
                               "(lower dense_max)");
     L.mfma = false;
     L.inv.release();
-    L.gj_f.release();
+    L.gj_w.release();
     L.gj_p.release();
     L.gj_bad.release();
   }
@@ -666,7 +666,7 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
 
 void Hierarchy::dense_inverse(Level& L, hipStream_t s) {
   launch_fill_i32(L.gj_bad.p, 0, 1, s);
-  launch_dense_inverse_mfma(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.inv.p, L.gj_f.p, L.gj_p.p, L.gj_bad.p, s);
+  launch_dense_inverse_mfma(L.n, L.rp.p, L.ci.p, L.v.p, L.pin.p, L.inv.p, L.gj_w.p, L.gj_p.p, L.gj_bad.p, s);
 }
 
 void Hierarchy::build_pad4(Level& L, hipStream_t s) {
@@ -695,10 +695,14 @@ void Hierarchy::build_pad4(Level& L, hipStream_t s) {
 void Hierarchy::refresh_f32(hipStream_t s) {
   for (int l = 0; l < (int)levels_.size(); ++l) {
     Level& L = levels_[l];
-    L.v32.alloc(L.nnz);
-    launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
     L.pad4 = false;
     if (!(l == 0 && stencil_)) build_pad4(L, s);
+    if (!L.pad4) {  // the unpadded FP32 values serve the levels without the padded form
+      L.v32.alloc(L.nnz);
+      launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    } else {
+      L.v32.release();
+    }
     if (!L.coarsest) {
       L.pv32.alloc(L.pnnz);
       L.rv32.alloc(L.pnnz);
@@ -745,8 +749,8 @@ void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
     Level& L = levels_[l];
     if (!(fine_done && l == 0)) {
       launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
-      launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
       if (L.pad4) launch_csr4_values(L.n, L.rp.p, L.v.p, L.rp4.p, L.v4.p, s);
+      else launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
     }
     if (L.coarsest) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);

@@ -98,13 +98,13 @@ void launch_gauss_jordan_inverse(int n, double* aug, int* done, hipStream_t s);
 // x = Ainv * b (with pin: b[0] treated as 0); FP64 and FP32 vector variants
 // Coarsest levels of more than kDenseSweepMax rows (up to kDenseMfmaMax): the inverse by
 // blocked Gauss-Jordan on the matrix cores (dense_mfma.hip) into M (dense_pad(n)^2
-// doubles), with F (dense_pad(n) x 64) and P (64 x 64) as work; *bad is set when a
+// doubles), with W (dense_pad(n)^2) and P (64 x 64) as work; *bad is set when a
 // pivot is not positive (the operator is not SPD). Applied by launch_dense_mv_*.
 constexpr int kDenseMfmaMax = 4096;
 constexpr int kDenseSweepMax = 160;  // coarsest levels up to this size: the register sweep
 int dense_pad(int n);
 void launch_dense_inverse_mfma(int n, const int* rp, const int* ci, const double* v, const int* pin, double* M,
-                               double* F, double* P, int* bad, hipStream_t s);
+                               double* W, double* P, int* bad, hipStream_t s);
 void launch_dense_mv_f64(int n, const int* pin, const double* M, const double* b, double* x, hipStream_t s);
 void launch_dense_mv_f32(int n, const int* pin, const double* M, const float* b, float* x, hipStream_t s);
 void launch_dense_solve_f64(int n, const int* pin, const double* aug, const double* b, double* x, hipStream_t s);

@@ -7,15 +7,20 @@
 // matrix and itself SPD: no pivoting is needed.
 //
 // The matrix (n x n, row-major) is padded to n_pad = 64 nb with an identity block.
-// For each block column k (one step of Gauss-Jordan on 64 columns at once), three
+// For each block column k (one step of Gauss-Jordan on 64 columns at once), two
 // launches:
 //   P = A_kk^-1                          one workgroup, the block in registers (4 x 4 pivot blocks)
-//   F_i <- A_ik (all i); A_kj <- P A_kj (j != k, MFMA); A_kk <- P
-//   A_ij <- A_ij - F_i A_kj (i, j != k); A_ik <- -F_i P (i != k)   MFMA
-// which is the scalar in-place elimination (a_kj /= a_kk; a_ij -= a_ik a_kj;
-// a_ik = -a_ik / a_kk; a_kk = 1 / a_kk) on 64 x 64 blocks; after the last block A
-// holds A^-1. Each visit of the V-cycle then applies the inverse as one dense
-// matrix-vector product (a wavefront per row, FP64 accumulation).
+//   from A into A':                      one workgroup per 64 x 64 block, MFMA
+//     A'_kk = P;  A'_kj = P A_kj;  A'_ik = -A_ik P;  A'_ij = A_ij - A_ik (P A_kj)
+// which is the scalar elimination (a_kj /= a_kk; a_ij -= a_ik a_kj; a_ik = -a_ik /
+// a_kk; a_kk = 1 / a_kk) on 64 x 64 blocks. Writing into a second matrix lets
+// every block read the old row and column panels, so each block forms its own
+// P A_kj instead of waiting on a row-panel launch. After the last block column
+// the inverse is in M (the elimination starts in W when nb is odd). Each visit of
+// the V-cycle then applies the inverse as one dense matrix-vector product (a
+// wavefront per row, FP64 accumulation).
+#include <utility>
+
 #include "amg_kernels.h"
 
 namespace impes {
@@ -63,26 +68,41 @@ __device__ __forceinline__ void load_tile(const double* __restrict__ g, long lon
 }
 
 // Dense n_pad x n_pad from CSR: one workgroup per row; the row is staged in LDS,
-// its entries added in entry order by one thread (repeated columns add up as in a
-// serial scatter), then written out. Padding rows are identity rows; a pinned DOF
-// 0 keeps only a_00 = 1 in its row and column.
-__global__ __launch_bounds__(256) void k_fill_padded(int n, int n_pad, const int* __restrict__ rp,
-                                                     const int* __restrict__ ci, const double* __restrict__ v,
-                                                     const int* __restrict__ pin_flag, double* __restrict__ M) {
+// its entries loaded by all threads a block at a time and added in entry order
+// by one thread (repeated columns add up as in a serial scatter), then written
+// out. Padding rows are identity rows; a pinned DOF 0 keeps only a_00 = 1 in its
+// row and column.
+constexpr int kFillThreads = 256;
+__global__ __launch_bounds__(kFillThreads) void k_fill_padded(int n, int n_pad, const int* __restrict__ rp,
+                                                              const int* __restrict__ ci, const double* __restrict__ v,
+                                                              const int* __restrict__ pin_flag, double* __restrict__ M) {
   extern __shared__ double row[];
+  __shared__ int sc[kFillThreads];
+  __shared__ double sv[kFillThreads];
   const int r = blockIdx.x;
   const int pin = *pin_flag;
   for (int c = threadIdx.x; c < n_pad; c += blockDim.x) row[c] = 0.0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (r >= n || (pin && r == 0)) {
-      row[r] = 1.0;
-    } else {
-      for (int e = rp[r]; e < rp[r + 1]; ++e) {
-        const int c = ci[e];
-        if (pin && c == 0) continue;
-        row[c] += v[e];
+  if (r >= n || (pin && r == 0)) {
+    __syncthreads();
+    if (threadIdx.x == 0) row[r] = 1.0;
+  } else {
+    const int e0 = rp[r], e1 = rp[r + 1];
+    for (int b = e0; b < e1; b += kFillThreads) {
+      const int e = b + threadIdx.x;
+      if (e < e1) {
+        sc[threadIdx.x] = ci[e];
+        sv[threadIdx.x] = v[e];
       }
+      __syncthreads();  // also orders the zeroing above before the first scatter
+      if (threadIdx.x == 0) {
+        const int m = min(kFillThreads, e1 - b);
+        for (int q = 0; q < m; ++q) {
+          const int c = sc[q];
+          if (pin && c == 0) continue;
+          row[c] += sv[q];
+        }
+      }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -179,60 +199,70 @@ __global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double*
   for (int m = 0; m < 16; ++m) P[(i0 + 4 * m) * kB + j] = a[m];
 }
 
-// Row panel, one workgroup per block index j: first F_j = A_jk (the old column
-// panel, which no workgroup of this launch writes), then A_kj <- P A_kj for j != k
-// and A_kk <- P.
-__global__ __launch_bounds__(kGemmThreads) void k_gj_row(int k, int n_pad, double* __restrict__ M,
-                                                         const double* __restrict__ P, double* __restrict__ F) {
-  __shared__ double As[kB * kLd], Bs[kB * kLd];
-  const int j = blockIdx.x;
-  const double* colblk = M + (long long)j * kB * n_pad + (long long)k * kB;
-  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) F[(long long)j * kB * kB + q] = colblk[(q / kB) * (long long)n_pad + q % kB];
-  if (j == k) {
-    __syncthreads();  // the copy of A_kk above is read before it is overwritten
-    for (int q = threadIdx.x; q < kB * kB; q += blockDim.x)
-      M[(long long)(k * kB + q / kB) * n_pad + (long long)k * kB + q % kB] = P[q];
-    return;
-  }
-  double* blk = M + (long long)k * kB * n_pad + (long long)j * kB;
-  load_tile(P, kB, As);
-  load_tile(blk, n_pad, Bs);
-  __syncthreads();
-  v4d acc[4];
-  tile_mma(As, Bs, acc);
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      int r, c;
-      tile_pos(t, e, r, c);
-      blk[(long long)r * n_pad + c] = acc[t][e];
-    }
-}
-
-// Trailing update of block (i, j), i != k: A_ij -= F_i A_kj (j != k) or
-// A_ik = -F_i P (j == k). One workgroup per block; blockIdx.x = j, blockIdx.y = i.
-__global__ __launch_bounds__(kGemmThreads) void k_gj_update(int k, int n_pad, double* __restrict__ M,
-                                                            const double* __restrict__ F,
-                                                            const double* __restrict__ P) {
+// One elimination step from A into A' (see the header), one workgroup per block
+// (blockIdx.x = j, blockIdx.y = i). Every block of column j != k forms P A_kj
+// itself with the same MFMA sequence, so the row panel it subtracts equals the one
+// its own row-k block stores.
+__global__ __launch_bounds__(kGemmThreads) void k_gj_step(int k, int n_pad, const double* __restrict__ A,
+                                                          double* __restrict__ An, const double* __restrict__ P) {
   __shared__ double As[kB * kLd], Bs[kB * kLd];
   const int j = blockIdx.x, i = blockIdx.y;
-  if (i == k) return;
-  load_tile(F + (long long)i * kB * kB, kB, As);
-  if (j == k) load_tile(P, kB, Bs);
-  else load_tile(M + (long long)k * kB * n_pad + (long long)j * kB, n_pad, Bs);
-  __syncthreads();
+  const double* blk = A + (long long)i * kB * n_pad + (long long)j * kB;
+  double* out = An + (long long)i * kB * n_pad + (long long)j * kB;
+  if (i == k && j == k) {
+    for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) out[(long long)(q / kB) * n_pad + q % kB] = P[q];
+    return;
+  }
   v4d acc[4];
-  tile_mma(As, Bs, acc);
-  double* blk = M + (long long)i * kB * n_pad + (long long)j * kB;
+  if (j == k) {  // A'_ik = -A_ik P
+    load_tile(blk, n_pad, As);
+    load_tile(P, kB, Bs);
+    __syncthreads();
+    tile_mma(As, Bs, acc);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int r, c;
+        tile_pos(t, e, r, c);
+        out[(long long)r * n_pad + c] = -acc[t][e];
+      }
+    return;
+  }
+  load_tile(P, kB, As);
+  load_tile(A + (long long)k * kB * n_pad + (long long)j * kB, n_pad, Bs);
+  __syncthreads();
+  tile_mma(As, Bs, acc);  // P A_kj
+  if (i == k) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int r, c;
+        tile_pos(t, e, r, c);
+        out[(long long)r * n_pad + c] = acc[t][e];
+      }
+    return;
+  }
+  __syncthreads();  // every wave is done reading As and Bs
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       int r, c;
       tile_pos(t, e, r, c);
-      double* dst = blk + (long long)r * n_pad + c;
-      *dst = (j == k) ? -acc[t][e] : *dst - acc[t][e];
+      Bs[r * kLd + c] = acc[t][e];
+    }
+  load_tile(A + (long long)i * kB * n_pad + (long long)k * kB, n_pad, As);
+  __syncthreads();
+  tile_mma(As, Bs, acc);  // A_ik (P A_kj)
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      int r, c;
+      tile_pos(t, e, r, c);
+      out[(long long)r * n_pad + c] = blk[(long long)r * n_pad + c] - acc[t][e];
     }
 }
 
@@ -267,18 +297,20 @@ __global__ __launch_bounds__(256) void k_dense_mv(int n, int n_pad, const int* _
 int dense_pad(int n) { return div_up(n, kB) * kB; }
 
 void launch_dense_inverse_mfma(int n, const int* rp, const int* ci, const double* v, const int* pin, double* M,
-                               double* F, double* P, int* bad, hipStream_t s) {
+                               double* W, double* P, int* bad, hipStream_t s) {
   IMPES_REQUIRE(n >= 1 && n <= kDenseMfmaMax, "MFMA dense inverse supports 1 <= n <= 4096");
   const int n_pad = dense_pad(n), nb = n_pad / kB;
-  k_fill_padded<<<n_pad, 256, n_pad * sizeof(double), s>>>(n, n_pad, rp, ci, v, pin, M);
+  // each step swaps the two matrices: start where nb steps end in M
+  double* cur = (nb % 2 == 0) ? M : W;
+  double* nxt = (nb % 2 == 0) ? W : M;
+  k_fill_padded<<<n_pad, kFillThreads, n_pad * sizeof(double), s>>>(n, n_pad, rp, ci, v, pin, cur);
   IMPES_LAUNCH_CHECK();
   for (int k = 0; k < nb; ++k) {
-    k_gj_diag<<<1, 256, 0, s>>>(k, n_pad, M, P, bad);
+    k_gj_diag<<<1, 256, 0, s>>>(k, n_pad, cur, P, bad);
     IMPES_LAUNCH_CHECK();
-    k_gj_row<<<nb, kGemmThreads, 0, s>>>(k, n_pad, M, P, F);
+    k_gj_step<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, cur, nxt, P);
     IMPES_LAUNCH_CHECK();
-    k_gj_update<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, M, F, P);
-    IMPES_LAUNCH_CHECK();
+    std::swap(cur, nxt);
   }
 }
 
