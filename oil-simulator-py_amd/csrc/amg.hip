@@ -803,6 +803,20 @@ void Hierarchy::smooth(int l, const T* b, T*& cur, T* other, int sweeps, bool& z
   Level& L = levels_[l];
   const bool st = (l == 0 && stencil_);
   for (int it = 0; it < sweeps; ++it) {
+    if (zero && st && it + 1 < sweeps) {
+      // the sweep from zero and the next one as one stencil pass
+      if constexpr (sizeof(T) == 8) {
+        Stencil7<double> S{nx_, ny_, nz_, sd_.p, sx_.p, sy_.p, sz_.p};
+        launch_st7_jacobi_z0_f64(S, (const double*)b, L.ir64.p, prm_.omega, (double*)other, s);
+      } else {
+        Stencil7<float> S{nx_, ny_, nz_, sd32_.p, sx32_.p, sy32_.p, sz32_.p};
+        launch_st7_jacobi_z0_f32(S, (const float*)b, L.ir32.p, (float)prm_.omega, (float*)other, s);
+      }
+      zero = false;
+      std::swap(cur, other);
+      ++it;
+      continue;
+    }
     if (zero) {
       Sel<T>::jac0(b, Sel<T>::ir(L), prm_.omega, cur, L.n, s);
       zero = false;
@@ -878,7 +892,10 @@ double Hierarchy::level_bytes(int l, int precision) const {
   const bool first_done = l > 0 && prm_.pre_smooth > 0;  // fused into the parent's restriction
   double b = 0.0;
   int pre = prm_.pre_smooth - (first_done ? 1 : 0);
-  if (!first_done && pre > 0) {
+  if (st && !first_done && pre > 1) {
+    b += n * vb * (4.0 + 3.0);  // the two sweeps from zero in one stencil pass: coefficients, b, inv_relax, out
+    pre -= 2;
+  } else if (!first_done && pre > 0) {
     b += 3.0 * n * vb;  // first sweep from zero: b, inv_relax -> x
     --pre;
   }

@@ -122,6 +122,12 @@ void launch_st7_jacobi_f64(const Stencil7<double>& S, const double* xi, const do
                            double w, double* xo, hipStream_t s);
 void launch_st7_jacobi_f32(const Stencil7<float>& S, const float* xi, const float* b, const float* inv_relax,
                            float w, float* xo, hipStream_t s);
+// two L1-Jacobi sweeps from a zero guess in one pass: with x1 = w * inv_relax .* b,
+// xo = x1 + w * inv_relax .* (b - A x1); equal to launch_jacobi0 then launch_st7_jacobi
+void launch_st7_jacobi_z0_f64(const Stencil7<double>& S, const double* b, const double* inv_relax, double w,
+                              double* xo, hipStream_t s);
+void launch_st7_jacobi_z0_f32(const Stencil7<float>& S, const float* b, const float* inv_relax, float w, float* xo,
+                              hipStream_t s);
 // y = dinv .* (A x)   (row-scaled operator used by the outer Krylov method)
 void launch_st7_rowscaled_spmv_f64(const Stencil7<double>& S, const double* dinv, const double* x, double* y, hipStream_t s);
 // z = sout .* x (x the FP32 or FP64 V-cycle output) and y = dinv .* (A z) in one pass

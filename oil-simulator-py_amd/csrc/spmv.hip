@@ -24,7 +24,12 @@ constexpr int kBlock = 256;
 // kRestrictJ0: y = A x (a restriction R r) and y2 = w * aux .* y (the first
 // L1-Jacobi sweep of the coarse level from a zero guess, aux = its inv_relax)
 // kYPlusB: y = b + A x (an add into another buffer; same arithmetic as kYAdd)
-enum Mode { kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5, kRestrictJ0 = 6, kYPlusB = 7 };
+// kJacZ0 (stencil): two L1-Jacobi sweeps from a zero guess in one pass: the first
+// sweep's x = w * aux .* b is formed at every stencil point instead of being read
+// (the expression of k_jacobi0), then y = x + w * aux .* (b - A x) as kJac
+enum Mode {
+  kY = 0, kYAdd = 1, kRes = 2, kJac = 3, kRowScaled = 4, kRowScaledRes = 5, kRestrictJ0 = 6, kYPlusB = 7, kJacZ0 = 8
+};
 
 template <int G, class T, class TA, int MODE>
 __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -229,6 +234,21 @@ struct St7Row {
     if (i < S.nx - 1) s += S.ox[c] * x[c + sx];
     return s;
   }
+  // the same sum with x[q] given by xv(q)
+  template <class F>
+  __device__ __forceinline__ static T apply_f(const Stencil7<T>& S, F xv, long long c, int i, int j, int k) {
+    const long long sy = S.nz;
+    const long long sx = (long long)S.ny * S.nz;
+    T s = T(0);
+    if (i > 0) s += S.ox[c - sx] * xv(c - sx);
+    if (j > 0) s += S.oy[c - sy] * xv(c - sy);
+    if (k > 0) s += S.oz[c - 1] * xv(c - 1);
+    s += S.d[c] * xv(c);
+    if (k < S.nz - 1) s += S.oz[c] * xv(c + 1);
+    if (j < S.ny - 1) s += S.oy[c] * xv(c + sy);
+    if (i < S.nx - 1) s += S.ox[c] * xv(c + sx);
+    return s;
+  }
 };
 
 template <class T, class TA, int MODE>
@@ -240,6 +260,12 @@ __global__ __launch_bounds__(kBlock) void k_st7(Stencil7<T> S, const T* __restri
     const long long t = c / S.nz;
     const int j = (int)(t % S.ny);
     const int i = (int)(t / S.ny);
+    if constexpr (MODE == kJacZ0) {
+      auto xv = [&](long long q) { return w * (T)aux[q] * b[q]; };
+      const T sum = St7Row<T>::apply_f(S, xv, c, i, j, k);
+      y[c] = xv(c) + w * (T)aux[c] * (b[c] - sum);
+      continue;
+    }
     const T sum = St7Row<T>::apply(S, x, c, i, j, k);
     if (MODE == kY) y[c] = sum;
     else if (MODE == kRes) y[c] = b[c] - sum;
@@ -289,20 +315,36 @@ __global__ __launch_bounds__(kBlock) void k_st7_wide(Stencil7<T> S, const T* __r
     const int i = (int)(t / S.ny);
     T xc[V], dd[V], fz[V], fy[V], fx[V];
     T fw[V] = {}, xw[V] = {}, fs[V] = {}, xs[V] = {}, xn[V] = {}, xe[V] = {}, bb[V] = {}, aa[V] = {};
-    ld_wide(x + c, xc);
+    // x at V aligned cells: read, or (kJacZ0) the first sweep w * aux .* b formed in place
+    auto xg = [&](long long q, T (&dst)[V]) {
+      if constexpr (MODE == kJacZ0) {
+        T bq[V], aq[V];
+        ld_wide(b + q, bq);
+        ld_wide(aux + q, aq);
+#pragma unroll
+        for (int m = 0; m < V; ++m) dst[m] = w * aq[m] * bq[m];
+      } else {
+        ld_wide(x + q, dst);
+      }
+    };
+    auto x1 = [&](long long q) {
+      if constexpr (MODE == kJacZ0) return w * aux[q] * b[q];
+      else return x[q];
+    };
+    xg(c, xc);
     ld_wide(S.d + c, dd);
     ld_wide(S.oz + c, fz);
     ld_wide(S.oy + c, fy);
     ld_wide(S.ox + c, fx);
-    if (i > 0) { ld_wide(S.ox + c - sx, fw); ld_wide(x + c - sx, xw); }
-    if (j > 0) { ld_wide(S.oy + c - sy, fs); ld_wide(x + c - sy, xs); }
-    if (j < S.ny - 1) ld_wide(x + c + sy, xn);
-    if (i < S.nx - 1) ld_wide(x + c + sx, xe);
+    if (i > 0) { ld_wide(S.ox + c - sx, fw); xg(c - sx, xw); }
+    if (j > 0) { ld_wide(S.oy + c - sy, fs); xg(c - sy, xs); }
+    if (j < S.ny - 1) xg(c + sy, xn);
+    if (i < S.nx - 1) xg(c + sx, xe);
     const T zb = k0 > 0 ? S.oz[c - 1] : T(0);
-    const T xb = k0 > 0 ? x[c - 1] : T(0);
-    const T xa = k0 + V < S.nz ? x[c + V] : T(0);
-    if (MODE == kRes || MODE == kJac || MODE == kRowScaledRes) ld_wide(b + c, bb);
-    if (MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes) ld_wide(aux + c, aa);
+    const T xb = k0 > 0 ? x1(c - 1) : T(0);
+    const T xa = k0 + V < S.nz ? x1(c + V) : T(0);
+    if (MODE == kRes || MODE == kJac || MODE == kRowScaledRes || MODE == kJacZ0) ld_wide(b + c, bb);
+    if (MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes || MODE == kJacZ0) ld_wide(aux + c, aa);
     T out[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) {
@@ -317,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_st7_wide(Stencil7<T> S, const T* __r
       if (i < S.nx - 1) sum += fx[m] * xe[m];
       if (MODE == kY) out[m] = sum;
       else if (MODE == kRes) out[m] = bb[m] - sum;
-      else if (MODE == kJac) out[m] = xc[m] + w * aa[m] * (bb[m] - sum);
+      else if (MODE == kJac || MODE == kJacZ0) out[m] = xc[m] + w * aa[m] * (bb[m] - sum);
       else if (MODE == kRowScaled) out[m] = aa[m] * sum;
       else out[m] = aa[m] * (bb[m] - sum);
     }
@@ -444,8 +486,8 @@ void st7_launch(const Stencil7<T>& S, const T* x, const T* b, const TA* aux, T w
   const long long n = (long long)S.nx * S.ny * S.nz;
   if (n <= 0) return;
   constexpr int V = 16 / sizeof(T);
-  constexpr bool uses_b = MODE == kRes || MODE == kJac || MODE == kRowScaledRes;
-  constexpr bool uses_aux = MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes;
+  constexpr bool uses_b = MODE == kRes || MODE == kJac || MODE == kRowScaledRes || MODE == kJacZ0;
+  constexpr bool uses_aux = MODE == kJac || MODE == kRowScaled || MODE == kRowScaledRes || MODE == kJacZ0;
   if constexpr (std::is_same<T, TA>::value) {
     // the wide form needs whole 16-byte vectors along z and 16-byte aligned arrays
     if (S.nz % V == 0 && aligned16(x) && aligned16(y) && aligned16(S.d) && aligned16(S.ox) && aligned16(S.oy) &&
@@ -596,6 +638,16 @@ void launch_st7_spmv_f64(const Stencil7<double>& S, const double* x, double* y, 
 }
 void launch_st7_residual_f64(const Stencil7<double>& S, const double* x, const double* b, double* r, hipStream_t s) {
   st7_launch<double, double, kRes>(S, x, b, (const double*)nullptr, 0.0, r, s);
+}
+void launch_st7_jacobi_z0_f64(const Stencil7<double>& S, const double* b, const double* inv_relax, double w,
+                              double* xo, hipStream_t s) {
+  KScope ks("st7_jacobi_z0_f64", st7_bytes(S, 8, 2), s);
+  st7_launch<double, double, kJacZ0>(S, nullptr, b, inv_relax, w, xo, s);
+}
+void launch_st7_jacobi_z0_f32(const Stencil7<float>& S, const float* b, const float* inv_relax, float w, float* xo,
+                              hipStream_t s) {
+  KScope ks("st7_jacobi_z0_f32", st7_bytes(S, 4, 2), s);
+  st7_launch<float, float, kJacZ0>(S, nullptr, b, inv_relax, w, xo, s);
 }
 void launch_st7_residual_f32(const Stencil7<float>& S, const float* x, const float* b, float* r, hipStream_t s) {
   KScope ks("st7_residual_f32", st7_bytes(S, 4, 2), s);
