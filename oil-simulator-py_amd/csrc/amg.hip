@@ -747,10 +747,17 @@ void Hierarchy::update_levels(bool fine_done, hipStream_t s) {
   }
   for (int l = 0; l < (int)levels_.size(); ++l) {
     Level& L = levels_[l];
-    if (!(fine_done && l == 0)) {
+    if (L.coarsest && L.dense) {
+      // the dense solve reads only the FP64 values: no FP32 copies (the weights stay
+      // current for the level's exported data)
       launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
-      if (L.pad4) launch_csr4_values(L.n, L.rp.p, L.v.p, L.rp4.p, L.v4.p, s);
-      else launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+    } else if (!(fine_done && l == 0)) {
+      if (L.pad4) {
+        launch_diag_relax_pad4(L.n, L.rp.p, L.ci.p, L.v.p, L.rp4.p, L.v4.p, L.ir64.p, L.ir32.p, s);
+      } else {
+        launch_diag_relax(L.n, L.rp.p, L.ci.p, L.v.p, nullptr, L.ir64.p, L.ir32.p, s);
+        launch_f64_to_f32(L.v32.p, L.v.p, L.nnz, s);
+      }
     }
     if (L.coarsest) {
       launch_pin_decide(L.n, L.rp.p, L.ci.p, L.v.p, L.pin_mode, L.pin.p, s);

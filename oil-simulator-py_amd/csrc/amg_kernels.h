@@ -12,6 +12,10 @@ constexpr signed char kF = 1;  // fine point marker
 // diag[i] = a_ii (0 if absent); inv_relax = min(1/max(|a_ii|, 0.3*sum_j|a_ij|, 1e-30), 1e2)
 void launch_diag_relax(int n, const int* rp, const int* ci, const double* v, double* diag, double* inv_relax64,
                        float* inv_relax32, hipStream_t s);
+// launch_diag_relax (diag not written) and the padded FP32 values v4 of the same
+// rows (as launch_csr4_values) in one pass
+void launch_diag_relax_pad4(int n, const int* rp, const int* ci, const double* v, const int* rp4, float* v4,
+                            double* ir64, float* ir32, hipStream_t s);
 // number of rows with |a_ii| < thr (lower-median test of the equilibration switch)
 void launch_count_small_diag(int n, const double* diag, double thr, int* flags, hipStream_t s);
 // dh[i] = 1/sqrt(max(|diag_i|, 1e-30))

@@ -86,6 +86,44 @@ __global__ void k_diag_relax(int n, const int* __restrict__ rp, const int* __res
   }
 }
 
+// k_diag_relax fused with the refresh of the 4-entry-padded FP32 values (v4,
+// rows starting at rp4, padding zeros): the same lanes walk the row once, in the
+// same entry order, so the relaxation weights equal k_diag_relax's.
+__global__ void k_diag_relax_pad4(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                  const double* __restrict__ v, const int* __restrict__ rp4, float* __restrict__ v4,
+                                  double* __restrict__ ir64, float* __restrict__ ir32) {
+  const int g = threadIdx.x % kDiagLanes;
+  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / kDiagLanes; i < n;
+       i += (long long)gridDim.x * blockDim.x / kDiagLanes) {
+    double d = 0.0, rs = 0.0;
+    int dpos = 0x7fffffff;
+    const int e0 = rp[i], len = rp[i + 1] - e0, q0 = rp4[i], len4 = rp4[i + 1] - q0;
+    for (int k = g; k < len4; k += kDiagLanes) {
+      float out = 0.f;
+      if (k < len) {
+        const int e = e0 + k;
+        const double a = v[e];
+        rs += fabs(a);
+        if (ci[e] == (int)i && e < dpos) { dpos = e; d = a; }
+        out = (float)a;
+      }
+      v4[q0 + k] = out;
+    }
+    rs = group_sum<kDiagLanes>(rs);
+    for (int off = kDiagLanes / 2; off > 0; off >>= 1) {
+      const int op = __shfl_xor(dpos, off, kWave);
+      const double od = __shfl_xor(d, off, kWave);
+      if (op < dpos) { dpos = op; d = od; }
+    }
+    if (g != 0) continue;
+    double den = fmax(fabs(d), 0.3 * rs);
+    den = fmax(den, 1e-30);
+    const double ir = fmin(1.0 / den, 1e2);
+    ir64[i] = ir;
+    ir32[i] = (float)ir;
+  }
+}
+
 __global__ void k_small_diag(int n, const double* __restrict__ diag, double thr, int* __restrict__ flags) {
   ROW_LOOP(i, n) flags[i] = (fmax(fabs(diag[i]), 1e-30) < thr) ? 1 : 0;
 }
@@ -912,6 +950,13 @@ void launch_diag_relax(int n, const int* rp, const int* ci, const double* v, dou
                        hipStream_t s) {
   if (n <= 0) return;
   k_diag_relax<<<grid_for((long long)n * kDiagLanes, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, v, diag, ir64, ir32);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_diag_relax_pad4(int n, const int* rp, const int* ci, const double* v, const int* rp4, float* v4,
+                            double* ir64, float* ir32, hipStream_t s) {
+  if (n <= 0) return;
+  k_diag_relax_pad4<<<grid_for((long long)n * kDiagLanes, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, v, rp4, v4,
+                                                                                          ir64, ir32);
   IMPES_LAUNCH_CHECK();
 }
 void launch_count_small_diag(int n, const double* diag, double thr, int* flags, hipStream_t s) {
