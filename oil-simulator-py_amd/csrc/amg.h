@@ -80,6 +80,8 @@ struct Level {
   int g4 = 1;
   DevBuf<int> rp4, ci4;  // ci4 released when the 16-bit offsets co4 fit
   DevBuf<short> co4;
+  int nw4 = 0;               // wave tiles of the padded operator (0: none)
+  DevBuf<int> wrow4, wq4;
   DevBuf<float> v4;
   DevBuf<signed char> cf;  // final C/F markers (non-coarsest levels)
   // interpolation / restriction to the next level

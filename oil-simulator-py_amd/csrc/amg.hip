@@ -251,7 +251,7 @@ struct Sel<float> {
   }
   static Csr4F32 A4(const Level& L) {
     return Csr4F32{L.n, L.nnz, L.rp4.p, reinterpret_cast<const int4*>(L.ci4.p), reinterpret_cast<const float4*>(L.v4.p),
-                   reinterpret_cast<const short4*>(L.co4.p)};
+                   reinterpret_cast<const short4*>(L.co4.p), L.nw4, L.wrow4.p, L.wq4.p};
   }
   static void jacL(const Level& L, const float* xi, const float* b, double w, float* xo, hipStream_t s) {
     if (L.pad4) launch_jacobi4_f32(A4(L), L.g4, xi, b, L.ir32.p, (float)w, xo, s);
@@ -690,6 +690,18 @@ void Hierarchy::build_pad4(Level& L, hipStream_t s) {
   launch_csr4_offsets(L.n, L.rp4.p, L.ci4.p, L.co4.p, maxabs.p, s);
   if (read_scalar(maxabs.p, s) <= 32767) L.ci4.release();  // 8.97 -> 8.88 ms per step at 1M cells
   else L.co4.release();
+  // wave tiles of whole rows (the pattern is fixed after setup)
+  std::vector<int> hrp(L.n + 1), wrow, wq;
+  IMPES_HIP_CHECK(hipMemcpyAsync(hrp.data(), L.rp4.p, sizeof(int) * (L.n + 1), hipMemcpyDeviceToHost, s));
+  IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  L.nw4 = csr4_wave_tiles(L.n, hrp.data(), wrow, wq);
+  if (L.nw4 > 0) {
+    L.wrow4.alloc(L.nw4 + 1);
+    L.wq4.alloc(L.nw4 + 1);
+    IMPES_HIP_CHECK(hipMemcpyAsync(L.wrow4.p, wrow.data(), sizeof(int) * (L.nw4 + 1), hipMemcpyHostToDevice, s));
+    IMPES_HIP_CHECK(hipMemcpyAsync(L.wq4.p, wq.data(), sizeof(int) * (L.nw4 + 1), hipMemcpyHostToDevice, s));
+    IMPES_HIP_CHECK(hipStreamSynchronize(s));
+  }
 }
 
 void Hierarchy::refresh_f32(hipStream_t s) {

@@ -2,6 +2,8 @@
 // All launchers enqueue on the given stream and never synchronise.
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 namespace impes {
@@ -69,7 +71,17 @@ struct Csr4F32 {
   // when set (every |column - row| < 2^15): 16-bit column offsets from the row,
   // read instead of ci4 (8 bytes per 4 entries instead of 16)
   const short4* co4 = nullptr;
+  // when set: the rows cut into nw wave tiles of whole rows and at most 64 chunks
+  // (tile t = rows wrow[t] .. wrow[t+1]-1, chunks wq[t] .. wq[t+1]-1)
+  int nw = 0;
+  const int* wrow = nullptr;
+  const int* wq = nullptr;
 };
+// Wave tiles of a padded operator (see Csr4F32): rows packed greedily, in order,
+// into tiles of at most 64 four-entry chunks. Fills wrow and wq (nw + 1 entries
+// each) from the host copy of rp4 (n + 1 entry offsets); returns nw, or 0 when a
+// row has more than 64 chunks.
+int csr4_wave_tiles(int n, const int* rp4_host, std::vector<int>& wrow, std::vector<int>& wq);
 // lanes per row of the padded kernels from the padded entry count
 int csr4_group_for(int n_rows, long long nnz4);
 // padded row lengths: cnt[r] = rows' lengths rounded up to 4

@@ -144,6 +144,34 @@ def test_hierarchy_matches_oracle(dev, case, mode):
     assert rel(xg32, xo) < 1e-4
 
 
+def test_padded_sweep_rows_longer_than_a_wave_tile(dev):
+    """A level with a row of more than 64 four-entry chunks (a hub cell coupled
+    to 400 others) cannot be cut into wave tiles of whole rows: that level's
+    FP32 sweeps take the lanes-per-row kernel, the others the tiled one. The
+    hierarchy and both V-cycles still match the oracle."""
+    import impes_hip as H
+
+    A, *_ = impes_like(16, 16, 6, seed=3)
+    A = sp.lil_matrix(A)
+    hub = np.random.default_rng(4).choice(np.arange(1, A.shape[0]), 400, replace=False)
+    c = 0.05 * abs(A[0, 0])
+    for j in hub:
+        A[0, j] = A[j, 0] = -c
+        A[j, j] = A[j, j] + c
+    A[0, 0] = A[0, 0] + c * len(hub)
+    A = sp.csr_matrix(A)
+    assert np.diff(A.indptr).max() > 256
+    kw = dict(theta=0.25, max_levels=10, coarsest_size=100)
+    ho = oa.Hierarchy(A, **kw)
+    hg = H.AmgHierarchy(**kw)
+    hg.setup_csr(*to_dev(A, dev))
+    _compare_hierarchy(hg, ho)
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    xo = ho.vcycle(b)
+    assert rel(hg.vcycle(t(b, dev), precision=64).cpu().numpy(), xo) < 1e-11
+    assert rel(hg.vcycle(t(b, dev), precision=32).cpu().numpy(), xo) < 1e-4
+
+
 @pytest.mark.parametrize("case", ["max_levels_1", "max_levels_2", "single_level_swept", "single_level_dense",
                                   "single_level_dense_negdef", "mfma_2342", "mfma_695", "mfma_1346_pinned",
                                   "mfma_single_4000"])
