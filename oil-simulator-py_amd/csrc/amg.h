@@ -65,6 +65,14 @@ struct LevelInfo {
   int n, nnz, p_nnz, is_coarsest, dense;
 };
 
+// 4-entry-padded FP32 copy of a grid transfer operator cut into wave
+// tiles of whole rows; nw == 0 when a row is too long for a tile
+struct Pad4 {
+  int n = 0, nnz = 0, nw = 0;
+  DevBuf<int> rp4, ci4, wrow, wq;
+  DevBuf<float> v4;
+};
+
 struct Level {
   int n = 0, nnz = 0, group = 1;
   bool coarsest = false, dense = false;
@@ -89,6 +97,7 @@ struct Level {
   DevBuf<int> prp, pci, rrp, rci;
   DevBuf<double> pv, rv;
   DevBuf<float> pv32, rv32;
+  Pad4 r4;  // FP32 restriction in the padded tiled form (read instead of rv32 when set)
   // Galerkin product data: AP = A*P (values refreshed on update), next A = R*AP
   int apnnz = 0;
   int ap_group = 1, ac_group = 1;  // lanes per output entry of the numeric products
@@ -207,6 +216,7 @@ class Hierarchy {
   void dense_inverse(Level& L, hipStream_t s);  // coarsest inverse on the matrix cores
   void refresh_f32(hipStream_t s);
   void build_pad4(Level& L, hipStream_t s);
+  void build_pad4_transfer(int rows, int nnz, const int* rp, const int* ci, const double* v, Pad4& T, hipStream_t s);
   void refresh_stencil(hipStream_t s);
   void alloc_vectors();
   // first_sweep_done: x already holds the first pre-smoothing sweep from zero

@@ -84,10 +84,13 @@ struct Csr4F32 {
 int csr4_wave_tiles(int n, const int* rp4_host, std::vector<int>& wrow, std::vector<int>& wq);
 // lanes per row of the padded kernels from the padded entry count
 int csr4_group_for(int n_rows, long long nnz4);
-// padded row lengths: cnt[r] = rows' lengths rounded up to 4
+// padded row lengths: cnt[r] = rows' lengths rounded up to 4 (an empty row gets 4)
 void launch_csr4_count(int n, const int* rp, int* cnt, hipStream_t s);
-// padded pattern (ci4 sized rp4[n]) and FP32 values (rounded from FP64) of a CSR matrix
-void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s);
+// padded pattern (ci4 sized rp4[n]) and FP32 values (rounded from FP64) of a CSR
+// matrix; padding entries have value 0 and the column of the row itself (square)
+// or of the row's first entry (rectangular; 0 for an empty row)
+void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s,
+                         bool square = true);
 void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s);
 // co4[e] = ci4[e] - row (16-bit) over nnz4 entries; *maxabs = the largest |ci4[e] - row|
 // (a device int, zero on entry)
@@ -96,6 +99,12 @@ void launch_csr4_offsets(int n, const int* rp4, const int* ci4, short* co4, int*
 void launch_residual4_f32(const Csr4F32& A, int group, const float* x, const float* b, float* r, hipStream_t s);
 void launch_jacobi4_f32(const Csr4F32& A, int group, const float* xi, const float* b, const float* inv_relax, float w,
                         float* xo, hipStream_t s);
+// restriction on a padded operator with wave tiles (A.wrow set; column indices in
+// ci4): y = A x; and y = A x with y2 = w inv_relax2 .* y (fused with the coarse
+// level's first sweep from zero)
+void launch_spmv4_f32(const Csr4F32& A, const float* x, float* y, hipStream_t s);
+void launch_restrict4_j0_f32(const Csr4F32& R, const float* r, const float* inv_relax_c, float w, float* bc, float* xc,
+                             hipStream_t s);
 
 // pick lanes-per-row from the mean row length
 int csr_group_for(int n_rows, int nnz);

@@ -60,7 +60,6 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
     if (active && g == 0) {
       if (MODE == kY) y[row] = sum;
       else if (MODE == kYAdd) y[row] = oy + sum;
-      else if (MODE == kYPlusB) y[row] = ob + sum;
       else if (MODE == kRes) y[row] = ob - sum;
       else if (MODE == kJac) y[row] = ox + w * oa * (ob - sum);
       else if (MODE == kRowScaled) y[row] = oa * sum;
@@ -135,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_csr4_tile(int nw, const int* __restr
                                                        const int* __restrict__ rp4, const int4* __restrict__ ci4,
                                                        const short4* __restrict__ co4, const float4* __restrict__ v4,
                                                        const float* __restrict__ x, const float* __restrict__ b,
-                                                       const float* __restrict__ aux, float w, float* __restrict__ y) {
+                                                       const float* __restrict__ aux, float w, float* __restrict__ y,
+                                                       float* __restrict__ y2) {
   constexpr int WPB = kBlock / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   for (long long t = xcd_block() * WPB + threadIdx.x / kWave; t < nw; t += (long long)gridDim.x * WPB) {
@@ -165,11 +165,9 @@ __global__ __launch_bounds__(kBlock) void k_csr4_tile(int nw, const int* __restr
     const bool last = valid && (lane == kWave - 1 || q + 1 >= q1 || rn != row);
     float ob = 0.f, ox = 0.f, oa = 0.f;
     if (last) {
-      ob = b[row];
-      if (MODE == kJac) {
-        ox = x[row];
-        oa = aux[row];
-      }
+      if (MODE == kRes || MODE == kJac) ob = b[row];
+      if (MODE == kJac) ox = x[row];
+      if (MODE == kJac || MODE == kRestrictJ0) oa = aux[row];
     }
     float sum = 0.f;
     if (valid) {
@@ -189,17 +187,23 @@ __global__ __launch_bounds__(kBlock) void k_csr4_tile(int nw, const int* __restr
     }
     if (last) {
       if (MODE == kRes) y[row] = ob - sum;
-      else y[row] = ox + w * oa * (ob - sum);
+      else if (MODE == kJac) y[row] = ox + w * oa * (ob - sum);
+      else if (MODE == kY) y[row] = sum;
+      else if (MODE == kYPlusB) y[row] = ob + sum;
+      else if (MODE == kRestrictJ0) {
+        y[row] = sum;
+        y2[row] = w * oa * sum;
+      }
     }
   }
 }
 
 template <int MODE, bool OFF16>
 void csr4_tile_launch(const Csr4F32& A, const float* x, const float* b, const float* aux, float w, float* y,
-                      hipStream_t s) {
+                      hipStream_t s, float* y2 = nullptr) {
   const int grid = grid_for(A.nw, kBlock / kWave, 1 << 18);
   klaunch(k_csr4_tile<MODE, OFF16>, dim3(grid), dim3(kBlock), 0, s, A.nw, A.wrow, A.wq, A.rp4, A.ci4, A.co4, A.v4, x, b,
-          aux, w, y);
+          aux, w, y, y2);
 }
 
 template <int MODE, bool OFF16>
@@ -245,14 +249,15 @@ void csr4_dispatch(const Csr4F32& A, int G, const float* x, const float* b, cons
 
 __global__ void k_csr4_count(int n, const int* __restrict__ rp, int* __restrict__ cnt) {
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x)
-    cnt[r] = (rp[r + 1] - rp[r] + 3) & ~3;
+    cnt[r] = max(4, (rp[r + 1] - rp[r] + 3) & ~3);
 }
 
 __global__ void k_csr4_pattern(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                               const int* __restrict__ rp4, int* __restrict__ ci4) {
+                               const int* __restrict__ rp4, int* __restrict__ ci4, int square) {
   for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < n; r += (long long)gridDim.x * blockDim.x) {
     const int e0 = rp[r], len = rp[r + 1] - e0, q0 = rp4[r], len4 = rp4[r + 1] - q0;
-    for (int k = 0; k < len4; ++k) ci4[q0 + k] = (k < len) ? ci[e0 + k] : (int)r;
+    const int pad = square ? (int)r : (len > 0 ? ci[e0] : 0);
+    for (int k = 0; k < len4; ++k) ci4[q0 + k] = (k < len) ? ci[e0 + k] : pad;
   }
 }
 
@@ -684,9 +689,9 @@ void launch_csr4_count(int n, const int* rp, int* cnt, hipStream_t s) {
   k_csr4_count<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, cnt);
   IMPES_LAUNCH_CHECK();
 }
-void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s) {
+void launch_csr4_pattern(int n, const int* rp, const int* ci, const int* rp4, int* ci4, hipStream_t s, bool square) {
   if (n <= 0) return;
-  k_csr4_pattern<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, rp4, ci4);
+  k_csr4_pattern<<<grid_for(n, kBlock, 1 << 16), kBlock, 0, s>>>(n, rp, ci, rp4, ci4, square ? 1 : 0);
   IMPES_LAUNCH_CHECK();
 }
 void launch_csr4_values(int n, const int* rp, const double* v, const int* rp4, float* v4, hipStream_t s) {
@@ -707,6 +712,19 @@ void launch_jacobi4_f32(const Csr4F32& A, int G, const float* xi, const float* b
                         float* xo, hipStream_t s) {
   KScope ks("csr_jacobi_f32", csr_bytes(A.n, A.nnz, 4, 3), s);
   csr4_dispatch<kJac>(A, G, xi, b, inv_relax, w, xo, s);
+}
+void launch_spmv4_f32(const Csr4F32& A, const float* x, float* y, hipStream_t s) {
+  IMPES_REQUIRE(A.wrow && !A.co4, "padded transfer: wave tiles and column indices required");
+  if (A.n <= 0) return;
+  csr4_tile_launch<kY, false>(A, x, nullptr, nullptr, 0.f, y, s);
+  IMPES_LAUNCH_CHECK();
+}
+void launch_restrict4_j0_f32(const Csr4F32& R, const float* r, const float* inv_relax_c, float w, float* bc, float* xc,
+                             hipStream_t s) {
+  IMPES_REQUIRE(R.wrow && !R.co4, "padded transfer: wave tiles and column indices required");
+  if (R.n <= 0) return;
+  csr4_tile_launch<kRestrictJ0, false>(R, r, nullptr, inv_relax_c, w, bc, s, xc);
+  IMPES_LAUNCH_CHECK();
 }
 void launch_restrict_j0_f64(const CsrF64& R, const double* r, const double* inv_relax_c, double w, double* bc,
                             double* xc, int G, hipStream_t s) {
