@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_csr(int n, const int* __restrict__ r
     if (active && g == 0) {
       if (MODE == kY) y[row] = sum;
       else if (MODE == kYAdd) y[row] = oy + sum;
+      else if (MODE == kYPlusB) y[row] = ob + sum;
       else if (MODE == kRes) y[row] = ob - sum;
       else if (MODE == kJac) y[row] = ox + w * oa * (ob - sum);
       else if (MODE == kRowScaled) y[row] = oa * sum;
@@ -189,7 +190,6 @@ __global__ __launch_bounds__(kBlock) void k_csr4_tile(int nw, const int* __restr
       if (MODE == kRes) y[row] = ob - sum;
       else if (MODE == kJac) y[row] = ox + w * oa * (ob - sum);
       else if (MODE == kY) y[row] = sum;
-      else if (MODE == kYPlusB) y[row] = ob + sum;
       else if (MODE == kRestrictJ0) {
         y[row] = sum;
         y2[row] = w * oa * sum;
