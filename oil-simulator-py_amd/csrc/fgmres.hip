@@ -504,6 +504,28 @@ __global__ void k_normalize_scale(double* v, const double* w, const double* __re
   }
 }
 
+// y = H^-1 g for the k x k upper-triangular H, one lane per row (k <= 64): the
+// triangle is staged in LDS by column with one round of loads, then column i
+// eliminates y_i from the rows above it (one shuffle, divide and multiply-add
+// per step, no global round trip on the chain).
+constexpr int kBackLanes = 64;
+__global__ __launch_bounds__(kBackLanes) void k_backsolve_wave(Layout L, int k, double* __restrict__ sm) {
+  __shared__ double hs[kBackLanes * kBackLanes];  // hs[i * 64 + j] = H(j, i), j <= i
+  const int lane = threadIdx.x;
+  for (int c = 0; c < k; ++c)
+    if (lane <= c) hs[c * kBackLanes + lane] = sm[L.H(lane, c)];
+  double gj = (lane < k) ? sm[L.g() + lane] : 0.0;
+  __syncthreads();
+  double* y = sm + L.y();
+  for (int i = k - 1; i >= 0; --i) {
+    const double gi = __shfl(gj, i, kWave);
+    const double d = hs[i * kBackLanes + i];
+    const double yi = (d != 0.0) ? gi / d : 0.0;
+    if (lane == i) y[i] = yi;
+    if (lane < i) gj -= hs[i * kBackLanes + lane] * yi;
+  }
+}
+
 __global__ void k_backsolve(Layout L, int k, double* __restrict__ sm) {
   const double* g = sm + L.g();
   double* y = sm + L.y();
@@ -745,7 +767,8 @@ void Fgmres::solve(const Operator& A, Hierarchy* M, int precision, const double*
         k_lag_colA<<<1, kColAThreads, 0, s>>>(L, k, sm, work_.p, kGsBlocks);
         IMPES_LAUNCH_CHECK();
       }
-      k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
+      if (k <= kBackLanes) k_backsolve_wave<<<1, kBackLanes, 0, s>>>(L, k, sm);
+      else k_backsolve<<<1, 1, 0, s>>>(L, k, sm);
       IMPES_LAUNCH_CHECK();
       launch_lincomb_add_f64(x, Z, n, k, sm + L.y(), n, s);
       residual(A, x, b, V, s);
