@@ -684,7 +684,7 @@ void Hierarchy::build_coarsest(int l, hipStream_t s) {
     const long long np = dense_pad(L.n);
     L.inv.alloc(np * np);
     L.gj_w.alloc(np * np);
-    L.gj_p.alloc(64 * 64);
+    L.gj_p.alloc(2 * 64 * 64);
     L.gj_bad.alloc(1);
     dense_inverse(L, s);
     if (read_scalar(L.gj_bad.p, s) == 0) return;

@@ -102,7 +102,7 @@ void launch_gauss_jordan_inverse(int n, double* aug, int* done, hipStream_t s);
 // x = Ainv * b (with pin: b[0] treated as 0); FP64 and FP32 vector variants
 // Coarsest levels of more than kDenseSweepMax rows (up to kDenseMfmaMax): the inverse by
 // blocked Gauss-Jordan on the matrix cores (dense_mfma.hip) into M (dense_pad(n)^2
-// doubles), with W (dense_pad(n)^2) and P (64 x 64) as work; *bad is set when a
+// doubles), with W (dense_pad(n)^2) and P (2 x 64 x 64) as work; *bad is set when a
 // pivot is not positive (the operator is not SPD). Applied by launch_dense_mv_*.
 constexpr int kDenseMfmaMax = 4096;
 constexpr int kDenseSweepMax = 160;  // coarsest levels up to this size: the register sweep

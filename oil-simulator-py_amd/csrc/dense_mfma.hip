@@ -11,6 +11,9 @@
 // launches:
 //   P = A_kk^-1                          one workgroup, the block in registers (4 x 4 pivot blocks)
 //   from A into A':                      one workgroup per 64 x 64 block, MFMA
+// where the first P comes from its own launch and each later one from the
+// workgroup of the previous step that produces that pivot block (so one launch
+// per block column after the first)
 //     A'_kk = P;  A'_kj = P A_kj;  A'_ik = -A_ik P;  A'_ij = A_ij - A_ik (P A_kj)
 // which is the scalar elimination (a_kj /= a_kk; a_ij -= a_ik a_kj; a_ik = -a_ik /
 // a_kk; a_kk = 1 / a_kk) on 64 x 64 blocks. Writing into a second matrix lets
@@ -120,14 +123,11 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_padded(int n, int n_pad, 
 // The reads of a step issue before the branch-free update that uses them. 16
 // barrier steps instead of 64: 88 -> 28 us per 64 x 64 block. A non-positive
 // pivot of a 4 x 4 inverse (not SPD) sets *bad.
-__global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double* __restrict__ M,
-                                                 double* __restrict__ P, int* __restrict__ bad) {
-  __shared__ double colb[2][kB][4], rowb[2][4][kB];
+// The inversion above on a block held as a[16] per thread (256 threads, thread t:
+// column t % 64, rows t / 64 + 4 m); colb and rowb are the exchange buffers.
+// Returns nonzero in a thread that met a non-positive pivot.
+__device__ __forceinline__ int invert_block(double (&a)[16], double (&colb)[2][kB][4], double (&rowb)[2][4][kB]) {
   const int j = threadIdx.x & (kB - 1), i0 = threadIdx.x >> 6;
-  const double* src = M + (long long)k * kB * n_pad + (long long)k * kB;
-  double a[16];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) a[m] = src[(long long)(i0 + 4 * m) * n_pad + j];
   int badp = 0;
   for (int q = 0; q < kB / 4; ++q) {
     const int buf = q & 1;
@@ -194,6 +194,18 @@ __global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double*
       a[m] = (m == q) ? R[i0] : outside;
     }
   }
+  return badp;
+}
+
+__global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double* __restrict__ M,
+                                                 double* __restrict__ P, int* __restrict__ bad) {
+  __shared__ double colb[2][kB][4], rowb[2][4][kB];
+  const int j = threadIdx.x & (kB - 1), i0 = threadIdx.x >> 6;
+  const double* src = M + (long long)k * kB * n_pad + (long long)k * kB;
+  double a[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) a[m] = src[(long long)(i0 + 4 * m) * n_pad + j];
+  const int badp = invert_block(a, colb, rowb);
   if (threadIdx.x == 0 && badp) *bad = 1;
 #pragma unroll
   for (int m = 0; m < 16; ++m) P[(i0 + 4 * m) * kB + j] = a[m];
@@ -204,8 +216,10 @@ __global__ __launch_bounds__(256) void k_gj_diag(int k, int n_pad, const double*
 // itself with the same MFMA sequence, so the row panel it subtracts equals the one
 // its own row-k block stores.
 __global__ __launch_bounds__(kGemmThreads) void k_gj_step(int k, int n_pad, const double* __restrict__ A,
-                                                          double* __restrict__ An, const double* __restrict__ P) {
+                                                          double* __restrict__ An, const double* __restrict__ P,
+                                                          double* __restrict__ Pn, int* __restrict__ bad) {
   __shared__ double As[kB * kLd], Bs[kB * kLd];
+  __shared__ double colb[2][kB][4], rowb[2][4][kB];
   const int j = blockIdx.x, i = blockIdx.y;
   const double* blk = A + (long long)i * kB * n_pad + (long long)j * kB;
   double* out = An + (long long)i * kB * n_pad + (long long)j * kB;
@@ -256,14 +270,30 @@ __global__ __launch_bounds__(kGemmThreads) void k_gj_step(int k, int n_pad, cons
   load_tile(A + (long long)i * kB * n_pad + (long long)k * kB, n_pad, As);
   __syncthreads();
   tile_mma(As, Bs, acc);  // A_ik (P A_kj)
+  const bool next_pivot = (i == k + 1 && j == k + 1 && Pn != nullptr);
+  if (next_pivot) __syncthreads();  // every wave is done reading As (block-uniform branch)
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       int r, c;
       tile_pos(t, e, r, c);
-      out[(long long)r * n_pad + c] = blk[(long long)r * n_pad + c] - acc[t][e];
+      const double v = blk[(long long)r * n_pad + c] - acc[t][e];
+      out[(long long)r * n_pad + c] = v;
+      if (next_pivot) As[r * kLd + c] = v;
     }
+  if (!next_pivot) return;
+  // this block is the next step's pivot block: invert it here (as k_gj_diag would
+  // from the stored block) and hand the inverse to step k + 1 in Pn
+  __syncthreads();
+  const int jj = threadIdx.x & (kB - 1), i0 = threadIdx.x >> 6;
+  double a[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) a[m] = As[(i0 + 4 * m) * kLd + jj];
+  const int badp = invert_block(a, colb, rowb);
+  if (threadIdx.x == 0 && badp) *bad = 1;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) Pn[(i0 + 4 * m) * kB + jj] = a[m];
 }
 
 // x = A^-1 b (b_0 taken as 0 when DOF 0 is pinned): a wavefront per row, four
@@ -305,10 +335,16 @@ void launch_dense_inverse_mfma(int n, const int* rp, const int* ci, const double
   double* nxt = (nb % 2 == 0) ? W : M;
   k_fill_padded<<<n_pad, kFillThreads, n_pad * sizeof(double), s>>>(n, n_pad, rp, ci, v, pin, cur);
   IMPES_LAUNCH_CHECK();
+  // P holds two 64 x 64 pivot inverses: step k reads P_k and its block (k+1, k+1)
+  // workgroup writes P_{k+1} into the other half
+  // (a separate pivot-inverse launch per block column took 7.72-7.73 against
+  // 7.67-7.70 ms per step at 1M cells)
+  k_gj_diag<<<1, 256, 0, s>>>(0, n_pad, cur, P, bad);
+  IMPES_LAUNCH_CHECK();
   for (int k = 0; k < nb; ++k) {
-    k_gj_diag<<<1, 256, 0, s>>>(k, n_pad, cur, P, bad);
-    IMPES_LAUNCH_CHECK();
-    k_gj_step<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, cur, nxt, P);
+    double* Pk = P + (k & 1) * kB * kB;
+    double* Pn = (k + 1 < nb) ? P + ((k + 1) & 1) * kB * kB : nullptr;
+    k_gj_step<<<dim3(nb, nb), kGemmThreads, 0, s>>>(k, n_pad, cur, nxt, Pk, Pn, bad);
     IMPES_LAUNCH_CHECK();
     std::swap(cur, nxt);
   }
