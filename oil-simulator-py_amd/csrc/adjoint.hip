@@ -170,8 +170,8 @@ __device__ __forceinline__ Dual<K> operator/(double a, const Dual<K>& b) {
 template <int K>
 __device__ __forceinline__ Dual<K> dpow(const Dual<K>& x, double e) {
   Dual<K> r;
-  r.v = pow(x.v, e);
-  const double c = e * pow(x.v, e - 1.0);
+  r.v = powe(x.v, e);  // the forward's value bit for bit
+  const double c = e * powe(x.v, e - 1.0);
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = (x.d[i] == 0.0) ? 0.0 : c * x.d[i];
   return r;

@@ -122,24 +122,33 @@ __device__ __forceinline__ double pvt_eval(const PvtDev& T, int key, double p_pa
 
 __device__ __forceinline__ double clamp01(double x) { return fmin(fmax(x, 0.0), 1.0); }
 
+// x^e for the curve exponents: the common Corey exponents 2 and 1 as the exact
+// product (pow goes through log and exp in FP64); the branch is uniform (e is a
+// fluid constant)
+__device__ __forceinline__ double powe(double x, double e) {
+  if (e == 2.0) return x * x;
+  if (e == 1.0) return x;
+  return pow(x, e);
+}
+
 // fluid.py:231-248, 377-435 -> kro, krw, krg
 __device__ __forceinline__ void relperms(const FluidParams& F, double sw, double sg, double& kro, double& krw,
                                          double& krg) {
   const double denom = 1.0 - F.sw_cr - F.so_r;
   const double sn = clamp01((sw - F.sw_cr) / denom);
-  krw = pow(sn, F.nw);
+  krw = powe(sn, F.nw);
   if (F.stone2) {
     const double swn = clamp01((sw - F.sw_cr) / (denom + 1e-12));
     const double dg = 1.0 - F.sg_cr - F.so_r;
     const double sgn = clamp01((sg - F.sg_cr) / (dg + 1e-12));
-    const double k = F.ko_end_w * pow(1.0 - swn, F.now_) + F.ko_end_g * pow(1.0 - sgn, F.nog) -
+    const double k = F.ko_end_w * powe(1.0 - swn, F.now_) + F.ko_end_g * powe(1.0 - sgn, F.nog) -
                      fmin(F.ko_end_w, F.ko_end_g);
     kro = fmax(k, 0.0);
   } else {
-    kro = pow(1.0 - sn, F.no);
+    kro = powe(1.0 - sn, F.no);
   }
   const double sg2 = clamp01(sg / (1.0 - F.sw_cr - F.so_r + 1e-10));
-  krg = pow(sg2, F.ng);
+  krg = powe(sg2, F.ng);
 }
 
 // fluid.py:273-287
