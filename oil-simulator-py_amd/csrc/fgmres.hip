@@ -8,6 +8,8 @@
 //   three Gram-Schmidt passes (dots; update + dots; update + norm + rotation),
 //   and the normalisation of the new basis vector, fused with the V-cycle's
 //   input scaling for the next step.
+#include <cstdint>
+
 #include "fgmres.h"
 
 #include <algorithm>
@@ -222,7 +224,7 @@ __device__ __forceinline__ T pair_sum(T v) {  // sum over the lanes of one colum
   return v;
 }
 
-template <int K, int P>
+template <int K, int P, int R>
 __global__ __launch_bounds__(kGsThreads) void k_lag_dots(int k, const double* __restrict__ V, long long n,
                                                          const double* __restrict__ u, const double* __restrict__ w,
                                                          double* __restrict__ part) {
@@ -236,18 +238,41 @@ __global__ __launch_bounds__(kGsThreads) void k_lag_dots(int k, const double* __
   for (int m = 0; m < KP; ++m) aa[m] = cc[m] = 0.0;
   double ss = 0.0, dd = 0.0;
   const long long rows_per_pass = (long long)gridDim.x * blockDim.x / P;
-  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
-    const double ui = u[i], wi = w[i];
+  if constexpr (R == 1) {
+    for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
+      const double ui = u[i], wi = w[i];
 #pragma unroll
-    for (int m = 0; m < KP; ++m) {
-      const int t = h + P * m;
-      const double v = (t < k) ? V[(long long)t * n + i] : 0.0;
-      aa[m] += v * ui;
-      cc[m] += v * wi;
+      for (int m = 0; m < KP; ++m) {
+        const int t = h + P * m;
+        const double v = (t < k) ? V[(long long)t * n + i] : 0.0;
+        aa[m] += v * ui;
+        cc[m] += v * wi;
+      }
+      if (h == 0) {
+        ss += ui * ui;
+        dd += ui * wi;
+      }
     }
-    if (h == 0) {
-      ss += ui * ui;
-      dd += ui * wi;
+  } else {  // rows 2 i2 and 2 i2 + 1 with 16-byte loads (n even, arrays 16-byte aligned)
+    for (long long i2 = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; 2 * i2 < n; i2 += rows_per_pass) {
+      const long long i = 2 * i2;
+      const double2 u2 = *reinterpret_cast<const double2*>(u + i), w2 = *reinterpret_cast<const double2*>(w + i);
+#pragma unroll
+      for (int m = 0; m < KP; ++m) {
+        const int t = h + P * m;
+        double2 v = make_double2(0.0, 0.0);
+        if (t < k) v = *reinterpret_cast<const double2*>(V + (long long)t * n + i);
+        aa[m] += v.x * u2.x;
+        aa[m] += v.y * u2.y;
+        cc[m] += v.x * w2.x;
+        cc[m] += v.y * w2.y;
+      }
+      if (h == 0) {
+        ss += u2.x * u2.x;
+        ss += u2.y * u2.y;
+        dd += u2.x * w2.x;
+        dd += u2.y * w2.y;
+      }
     }
   }
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -280,7 +305,7 @@ __global__ __launch_bounds__(kGsThreads) void k_lag_dots(int k, const double* __
   }
 }
 
-template <int K, int P, class TO>
+template <int K, int P, class TO, int R>
 __global__ __launch_bounds__(kGsThreads) void k_lag_update(int k, double* __restrict__ V, long long n,
                                                            const double* __restrict__ w,
                                                            const double* __restrict__ ca,
@@ -304,28 +329,65 @@ __global__ __launch_bounds__(kGsThreads) void k_lag_update(int k, double* __rest
   double acc = 0.0;
   const long long rows_per_pass = (long long)gridDim.x * blockDim.x / P;
   // every lane of a pair walks the same rows (the pair folds with one shuffle)
-  for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
-    double su = 0.0, sw = 0.0;
+  if constexpr (R == 1) {
+    for (long long i = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; i < n; i += rows_per_pass) {
+      double su = 0.0, sw = 0.0;
 #pragma unroll
-    for (int m = 0; m < KP; ++m) {
-      const int t = h + P * m;
-      if (t < k) {
-        const double v = V[(long long)t * n + i];
-        su += v * a[m];
-        sw += v * c[m];
+      for (int m = 0; m < KP; ++m) {
+        const int t = h + P * m;
+        if (t < k) {
+          const double v = V[(long long)t * n + i];
+          su += v * a[m];
+          sw += v * c[m];
+        }
+      }
+      if (P > 1) {
+        su += __shfl_xor(su, 1, kWave);
+        sw += __shfl_xor(sw, 1, kWave);
+      }
+      if (h == 0) {
+        const double vj = (u[i] - su) * inv_nu;
+        const double w1 = (w[i] - sw) - vj * e;
+        u[i] = vj;
+        un[i] = w1;
+        bout[i] = (TO)(S[i] * w1);
+        acc += w1 * w1;
       }
     }
-    if (P > 1) {
-      su += __shfl_xor(su, 1, kWave);
-      sw += __shfl_xor(sw, 1, kWave);
-    }
-    if (h == 0) {
-      const double vj = (u[i] - su) * inv_nu;
-      const double w1 = (w[i] - sw) - vj * e;
-      u[i] = vj;
-      un[i] = w1;
-      bout[i] = (TO)(S[i] * w1);
-      acc += w1 * w1;
+  } else {  // rows 2 i2 and 2 i2 + 1 with 16-byte loads; each row's arithmetic as above
+    for (long long i2 = (blockIdx.x * (long long)blockDim.x + threadIdx.x) / P; 2 * i2 < n; i2 += rows_per_pass) {
+      const long long i = 2 * i2;
+      double sux = 0.0, suy = 0.0, swx = 0.0, swy = 0.0;
+#pragma unroll
+      for (int m = 0; m < KP; ++m) {
+        const int t = h + P * m;
+        if (t < k) {
+          const double2 v = *reinterpret_cast<const double2*>(V + (long long)t * n + i);
+          sux += v.x * a[m];
+          suy += v.y * a[m];
+          swx += v.x * c[m];
+          swy += v.y * c[m];
+        }
+      }
+      if (P > 1) {
+        sux += __shfl_xor(sux, 1, kWave);
+        suy += __shfl_xor(suy, 1, kWave);
+        swx += __shfl_xor(swx, 1, kWave);
+        swy += __shfl_xor(swy, 1, kWave);
+      }
+      if (h == 0) {
+        const double2 u2 = *reinterpret_cast<const double2*>(u + i);
+        const double2 w2 = *reinterpret_cast<const double2*>(w + i);
+        const double2 s2 = *reinterpret_cast<const double2*>(S + i);
+        const double vjx = (u2.x - sux) * inv_nu, vjy = (u2.y - suy) * inv_nu;
+        const double w1x = (w2.x - swx) - vjx * e, w1y = (w2.y - swy) - vjy * e;
+        *reinterpret_cast<double2*>(u + i) = make_double2(vjx, vjy);
+        *reinterpret_cast<double2*>(un + i) = make_double2(w1x, w1y);
+        bout[i] = (TO)(s2.x * w1x);
+        bout[i + 1] = (TO)(s2.y * w1y);
+        acc += w1x * w1x;
+        acc += w1y * w1y;
+      }
     }
   }
   acc = block_sum(acc, lds);
@@ -470,14 +532,27 @@ template <int K>
 void launch_lag_pair(int k, double* V, long long n, const double* w, double* sm, Layout L, const double* S, void* bout,
                      int precision, double* part, bool dots_only, const double* u, hipStream_t s) {
   constexpr int P = K >= 16 ? 2 : 1;  // two lanes per row where the accumulators would crowd the registers
+  // two rows per lane with 16-byte loads where n is even and the arrays aligned:
+  // half the load instructions for the same bytes (7.70 -> 7.60 ms per step at 1M cells)
+  auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool two = n % 2 == 0 && a16(V) && a16(w) && (dots_only ? a16(u) : (a16(S) && bout != nullptr));
   if (dots_only) {
-    k_lag_dots<K, P><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, u, w, part);
+    if (two) k_lag_dots<K, P, 2><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, u, w, part);
+    else k_lag_dots<K, P, 1><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, u, w, part);
   } else if (precision == 32) {
-    k_lag_update<K, P, float><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(), S,
-                                                               (float*)bout, part);
+    if (two)
+      k_lag_update<K, P, float, 2><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(),
+                                                                    S, (float*)bout, part);
+    else
+      k_lag_update<K, P, float, 1><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(),
+                                                                    S, (float*)bout, part);
   } else {
-    k_lag_update<K, P, double><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(), sm + L.lag(),
-                                                                S, (double*)bout, part);
+    if (two)
+      k_lag_update<K, P, double, 2><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(),
+                                                                     sm + L.lag(), S, (double*)bout, part);
+    else
+      k_lag_update<K, P, double, 1><<<kGsBlocks, kGsThreads, 0, s>>>(k, V, n, w, sm + L.h(), sm + L.h2(),
+                                                                     sm + L.lag(), S, (double*)bout, part);
   }
   IMPES_LAUNCH_CHECK();
 }
