@@ -93,6 +93,7 @@ class Simulator:
         self.step_log = []
         self._p_prev = None
         self._p_hist = None  # (pressure at the start of the last accepted step, its dt)
+        self._p_hist2 = None  # the same for the accepted step before it
 
     # ------------------------------------------------------------ wells
     def _ijk_to_idx(self, i, j, k, nx=None, ny=None):
@@ -149,6 +150,7 @@ class Simulator:
         for _attempt in range(max_attempts):
             P_new, converged = self._impes_pressure_step(current_dt)
             if converged:
+                self._p_hist2 = self._p_hist
                 self._p_hist = (self._p_prev.clone(), current_dt)
                 self.fluid.pressure = P_new
                 self._impes_saturation_step(P_new, current_dt)
@@ -199,16 +201,24 @@ class Simulator:
         accepted step, p + (p - p_last) dt / dt_last, when the last step was at
         least as long as this one; after a shorter step (one the attempt loop cut)
         the current pressure, since the change over a cut step says little about
-        the next full one. Either way the solve runs to the same stopping test; a
-        better start saves iterations."""
+        the next full one. "extrapolate2": with three equal steps behind it,
+        p_last + (p - p_last2), the change over two steps added to the pressure
+        two steps back, which is exact for a linear trend plus a component that
+        alternates from step to step (as the pressure of cases with unconverted
+        surface-rate injectors does); otherwise as "extrapolate". Either way the
+        solve runs to the same stopping test; a better start saves iterations."""
         mode = str(cfg.get("initial_guess", "extrapolate"))
         if mode == "zero":
             return torch.zeros_like(p)
-        if mode != "extrapolate" or self._p_hist is None:
+        if mode not in ("extrapolate", "extrapolate2") or self._p_hist is None:
             return p
         p_last, dt_last = self._p_hist
         if p_last.shape != p.shape or not float(dt_last) >= float(dt):
             return p
+        if mode == "extrapolate2" and self._p_hist2 is not None:
+            p_last2, dt_last2 = self._p_hist2
+            if p_last2.shape == p.shape and float(dt_last) == float(dt) and float(dt_last2) == float(dt):
+                return p_last + (p - p_last2)
         return torch.add(p, p - p_last, alpha=float(dt) / float(dt_last))
 
     def _impes_saturation_step(self, P_new, dt):
